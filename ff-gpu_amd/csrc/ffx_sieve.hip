@@ -1,0 +1,395 @@
+// libffx sieve: the odd-only prime bitmap, written by LDS-resident segments.
+//
+// Replaces SieveSlab_<arch> (source/sieve_slab_kernel.h:40-107), its launch
+// and prime cache (source/sieve_slab_engine.cpp:115-202, :279-312, :409-442)
+// and the host small-prime generation of SieveEngine
+// (source/sieve_engine.cpp:31-47, :114-133).  The bitmap it writes is the
+// one Prime::Prime + SegmentFill build (reference/segmentedSieve.C:177-222,
+// :255-270): odd n in byte n>>4 under mask 0x80>>((n>>1)&7), 1 = prime for
+// odd n <= M, untouched (1) for odd n > M, value 1 cleared.
+//
+// Kernel design (gfx950, wave64, 160 KiB LDS per CU):
+//  * One workgroup of 1024 threads owns one 64 KiB map segment at a time
+//    (2^19 odd values) in LDS and walks segments in a grid-stride loop.
+//  * Bits are kept "struck" = 1 in LDS, in natural order (bit b of word w is
+//    odd index 32w+b); the write-out inverts them and reorders each byte to
+//    the reference's MSB-first layout (bit position b^7 within the word).
+//  * Primes 3..61 never strike: their multiples are OR-ed in as periodic word
+//    patterns (eight groups, periods 105..3599 words) kept in LDS.
+//  * Primes 67..~8191 (more than 64 strikes per segment) are struck
+//    cooperatively: one wavefront per prime, lane l striking every 64th
+//    multiple.  Larger primes are struck one prime per lane.  Strikes are
+//    LDS atomic ORs (ds_or_b32).
+//  * The segment leaves LDS through 16-byte coalesced stores.
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "ffx_internal.h"
+
+namespace {
+
+constexpr uint32_t kSegBytes = 65536;
+constexpr uint32_t kSegWords = kSegBytes / 4;
+constexpr uint32_t kSegOdds = kSegBytes * 8;  // odd values per segment
+constexpr int kThreads = 1024;
+constexpr int kWaves = kThreads / 64;
+constexpr uint32_t kCoopBelow = kSegOdds / 64;  // primes below strike wave-cooperatively
+
+constexpr int kGroups = 8;
+// Small-prime groups whose word patterns replace strikes.  Each period (the
+// product of the group's primes) is coprime to 32, so the pattern of odd
+// indices repeats every `period` 32-bit words.
+constexpr uint32_t kGroupPrimes[kGroups][3] = {{3, 5, 7}, {11, 13, 0}, {17, 19, 0}, {23, 29, 0},
+                                               {31, 37, 0}, {41, 43, 0}, {47, 53, 0}, {59, 61, 0}};
+constexpr uint32_t kLargestPatternPrime = 61;
+
+constexpr uint32_t group_period(int g)
+{
+    return kGroupPrimes[g][0] * kGroupPrimes[g][1] * (kGroupPrimes[g][2] ? kGroupPrimes[g][2] : 1);
+}
+constexpr uint32_t pattern_words()
+{
+    uint32_t s = 0;
+    for (int g = 0; g < kGroups; ++g) s += group_period(g);
+    return s;
+}
+constexpr uint32_t kPatWords = pattern_words();  // 10238 words, 40 KiB
+
+struct SieveArgs {
+    uint8_t *dst;          // receives map bytes [byteLo, byteHi) at dst[byte - dstBase]
+    uint64_t dstBase;
+    uint64_t byteLo, byteHi;
+    uint64_t oddLimit;     // odd indices j >= oddLimit (values > M) are never struck
+    const uint32_t *primes;  // sieving primes > 61, ascending
+    const double *invp;      // 1.0 / primes[i]
+    uint32_t nprimes;
+    uint32_t ncoop;          // primes[0, ncoop) strike wave-cooperatively
+    const uint32_t *patterns;  // kPatWords words, groups back to back
+    uint64_t segFirst, segCount;
+};
+
+// First odd index >= j0 struck by prime p (its odd multiples from p*p on,
+// i.e. odd indices congruent to (p-1)/2 mod p, from (p*p-1)/2), relative to
+// j0.  j0 < 2^52 so the double quotient is off by at most one.
+__device__ __forceinline__ uint64_t first_strike(uint64_t j0, uint32_t p, double invp)
+{
+    int64_t q = (int64_t)((double)j0 * invp);
+    int64_t m = (int64_t)j0 - q * (int64_t)p;
+    if (m < 0) m += p;
+    if (m >= (int64_t)p) m -= p;
+    uint32_t r = (p - 1) >> 1;
+    uint64_t off = (uint32_t)m <= r ? r - (uint32_t)m : r + p - (uint32_t)m;
+    uint64_t sq = ((uint64_t)p * p - 1) >> 1;
+    if (j0 + off < sq) off = sq - j0;
+    return off;
+}
+
+// Reorders a word of natural-order bits (bit b = odd index b) into the
+// reference byte layout: within each little-endian byte the smallest odd
+// value is the most significant bit, i.e. bit b moves to b ^ 7.
+__device__ __forceinline__ uint32_t to_map_order(uint32_t v)
+{
+    return __builtin_bswap32(__builtin_bitreverse32(v));
+}
+
+__global__ __launch_bounds__(kThreads) void sieve_segments(SieveArgs a)
+{
+    __shared__ uint32_t seg[kSegWords];
+    __shared__ uint32_t pat[kPatWords];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+
+    for (uint32_t i = tid; i < kPatWords; i += kThreads) pat[i] = a.patterns[i];
+
+    // Per-thread pattern cursors: word index (segWord0 + tid) mod period,
+    // advanced by kThreads words per fill step and by the grid stride per
+    // segment, so no division runs inside the loops.
+    uint32_t period[kGroups], base[kGroups], stepFill[kGroups], stepSeg[kGroups], cur[kGroups];
+    uint32_t off = 0;
+    {
+        const uint64_t w0 = (a.segFirst + blockIdx.x) * (uint64_t)kSegWords + tid;
+        const uint64_t segStride = (uint64_t)gridDim.x * kSegWords;
+#pragma unroll
+        for (int g = 0; g < kGroups; ++g) {
+            period[g] = group_period(g);
+            base[g] = off;
+            off += period[g];
+            cur[g] = (uint32_t)(w0 % period[g]);
+            stepFill[g] = kThreads % period[g];
+            stepSeg[g] = (uint32_t)(segStride % period[g]);
+        }
+    }
+    __syncthreads();
+
+    for (uint64_t si = blockIdx.x; si < a.segCount; si += gridDim.x) {
+        const uint64_t segIdx = a.segFirst + si;
+        const uint64_t j0 = segIdx * (uint64_t)kSegOdds;
+        const uint64_t segByte0 = segIdx * (uint64_t)kSegBytes;
+
+        // 1. small-prime patterns
+        {
+            uint32_t c[kGroups];
+#pragma unroll
+            for (int g = 0; g < kGroups; ++g) c[g] = cur[g];
+#pragma unroll 4
+            for (uint32_t w = tid; w < kSegWords; w += kThreads) {
+                uint32_t v = 0;
+#pragma unroll
+                for (int g = 0; g < kGroups; ++g) {
+                    v |= pat[base[g] + c[g]];
+                    c[g] += stepFill[g];
+                    if (c[g] >= period[g]) c[g] -= period[g];
+                }
+                seg[w] = v;
+            }
+        }
+        __syncthreads();
+
+        // 2. medium primes
+        const uint64_t limit64 = a.oddLimit > j0 ? a.oddLimit - j0 : 0;
+        const uint32_t limit = limit64 < kSegOdds ? (uint32_t)limit64 : kSegOdds;
+        if (limit) {
+            for (uint32_t pi = wave; pi < a.ncoop; pi += kWaves) {
+                const uint32_t p = a.primes[pi];
+                const uint64_t first = first_strike(j0, p, a.invp[pi]);
+                if (first >= limit) continue;
+                const uint32_t stride = 64u * p;
+                for (uint32_t j = (uint32_t)first + (uint32_t)lane * p; j < limit; j += stride)
+                    atomicOr(&seg[j >> 5], 1u << (j & 31));
+            }
+            for (uint32_t pi = a.ncoop + tid; pi < a.nprimes; pi += kThreads) {
+                const uint32_t p = a.primes[pi];
+                const uint64_t first = first_strike(j0, p, a.invp[pi]);
+                if (first >= limit) continue;
+                for (uint32_t j = (uint32_t)first; j < limit; j += p) atomicOr(&seg[j >> 5], 1u << (j & 31));
+            }
+        }
+        __syncthreads();
+
+        // 3. write-out: 16 bytes (4 words) per thread per step
+        const uint64_t lo = a.byteLo > segByte0 ? a.byteLo : segByte0;
+        const uint64_t hiSeg = segByte0 + kSegBytes;
+        const uint64_t hi = a.byteHi < hiSeg ? a.byteHi : hiSeg;
+        for (uint32_t ch = tid; ch < kSegBytes / 16; ch += kThreads) {
+            const uint64_t b0 = segByte0 + 16ull * ch;
+            if (b0 + 16 <= lo || b0 >= hi) continue;
+            uint32_t out[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t w = 4 * ch + k;
+                uint32_t v = seg[w];
+                const uint64_t jw = j0 + 32ull * w;  // odd index of bit 0
+                if (jw == 0) {
+                    // value 1 is not prime; the pattern primes themselves are
+                    v |= 1u;
+                    for (int g = 0; g < kGroups; ++g)
+                        for (int t = 0; t < 3; ++t)
+                            if (kGroupPrimes[g][t]) v &= ~(1u << ((kGroupPrimes[g][t] - 1) >> 1));
+                }
+                if (jw + 32 > a.oddLimit) {
+                    // values above M keep their initial 1 (never struck)
+                    const uint64_t keep = a.oddLimit > jw ? a.oddLimit - jw : 0;
+                    v &= keep >= 32 ? 0xFFFFFFFFu : ((1u << keep) - 1u);
+                }
+                out[k] = to_map_order(~v);
+            }
+            uint8_t *d = a.dst + (b0 - a.dstBase);
+            if (b0 >= lo && b0 + 16 <= hi && ((uintptr_t)d & 15) == 0) {
+                *reinterpret_cast<uint4 *>(d) = make_uint4(out[0], out[1], out[2], out[3]);
+            } else {
+                const uint8_t *ob = reinterpret_cast<const uint8_t *>(out);
+                for (int k = 0; k < 16; ++k)
+                    if (b0 + k >= lo && b0 + k < hi) d[k] = ob[k];
+            }
+        }
+        __syncthreads();
+
+#pragma unroll
+        for (int g = 0; g < kGroups; ++g) {
+            cur[g] += stepSeg[g];
+            if (cur[g] >= period[g]) cur[g] -= period[g];
+        }
+    }
+}
+
+uint64_t isqrt_u64(uint64_t n)
+{
+    uint64_t r = (uint64_t)std::sqrt((double)n);
+    if (r > 0xFFFFFFFFull) r = 0xFFFFFFFFull;
+    while (r * r > n) --r;
+    while (r < 0xFFFFFFFFull && (r + 1) * (r + 1) <= n) ++r;
+    return r;
+}
+
+}  // namespace
+
+struct ffx_sieve_plan {
+    uint64_t maxPrimeMapValue;
+    uint32_t nprimes, ncoop;
+    uint32_t *dPrimes;
+    double *dInv;
+    uint32_t *dPatterns;
+    int device;
+    int grid;
+};
+
+using namespace ffx;
+
+static int plan_build(ffx_sieve_plan **out, std::vector<uint32_t> &oddPrimes, uint64_t M)
+{
+    if (!out) return fail(FFX_EINVAL, "sieve plan: null out");
+    *out = nullptr;
+    // keep the primes the kernel strikes: 61 < p <= isqrt(M)
+    const uint64_t sq = isqrt_u64(M);
+    std::vector<uint32_t> med;
+    for (uint32_t p : oddPrimes)
+        if (p > kLargestPatternPrime && p <= sq) med.push_back(p);
+    std::vector<double> inv(med.size());
+    uint32_t ncoop = 0;
+    for (size_t i = 0; i < med.size(); ++i) {
+        inv[i] = 1.0 / (double)med[i];
+        if (med[i] < kCoopBelow) ncoop = (uint32_t)i + 1;
+    }
+    // word patterns: bit b of word i of group g is set when odd index 32i+b
+    // is congruent to (p-1)/2 mod some prime p of the group
+    std::vector<uint32_t> pat(kPatWords, 0);
+    uint32_t off = 0;
+    for (int g = 0; g < kGroups; ++g) {
+        const uint32_t P = group_period(g);
+        for (uint64_t j = 0; j < 32ull * P; ++j)
+            for (int t = 0; t < 3; ++t) {
+                const uint32_t p = kGroupPrimes[g][t];
+                if (p && j % p == (p - 1) / 2) pat[off + j / 32] |= 1u << (j % 32);
+            }
+        off += P;
+    }
+    ffx_sieve_plan *pl = new (std::nothrow) ffx_sieve_plan();
+    if (!pl) return fail(FFX_ENOMEM, "sieve plan: host alloc");
+    pl->maxPrimeMapValue = M;
+    pl->nprimes = (uint32_t)med.size();
+    pl->ncoop = ncoop;
+    hipError_t e = hipGetDevice(&pl->device);
+    int cus = 0;
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, pl->device);
+    if (e == hipSuccess) e = hipMalloc(&pl->dPrimes, (med.size() + 1) * 4);
+    if (e == hipSuccess) e = hipMalloc(&pl->dInv, (med.size() + 1) * 8);
+    if (e == hipSuccess) e = hipMalloc(&pl->dPatterns, kPatWords * 4);
+    if (e == hipSuccess && !med.empty()) e = hipMemcpy(pl->dPrimes, med.data(), med.size() * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess && !med.empty()) e = hipMemcpy(pl->dInv, inv.data(), inv.size() * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(pl->dPatterns, pat.data(), kPatWords * 4, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        int rc = hip_fail(e, "sieve plan upload");
+        ffx_sieve_plan_destroy(pl);
+        return rc;
+    }
+    pl->grid = cus > 0 ? cus : 256;  // one 1024-thread group per CU (LDS-bound)
+    *out = pl;
+    return FFX_OK;
+}
+
+static int launch(const ffx_sieve_plan *pl, uint8_t *dst, uint64_t dstBase, uint64_t byteLo, uint64_t byteHi,
+                  void *stream)
+{
+    if (!pl) return fail(FFX_EINVAL, "sieve: null plan");
+    if (!dst) return fail(FFX_EINVAL, "sieve: null destination");
+    if (byteHi <= byteLo) return FFX_OK;
+    SieveArgs a;
+    a.dst = dst;
+    a.dstBase = dstBase;
+    a.byteLo = byteLo;
+    a.byteHi = byteHi;
+    a.oddLimit = (pl->maxPrimeMapValue + 1) >> 1;
+    a.primes = pl->dPrimes;
+    a.invp = pl->dInv;
+    a.nprimes = pl->nprimes;
+    a.ncoop = pl->ncoop;
+    a.patterns = pl->dPatterns;
+    a.segFirst = byteLo / kSegBytes;
+    a.segCount = (byteHi + kSegBytes - 1) / kSegBytes - a.segFirst;
+    if (a.segFirst * (uint64_t)kSegOdds >= (1ull << 52))
+        return fail(FFX_ERANGE, "sieve: map offset beyond 2^53 values");
+    const uint32_t grid = a.segCount < (uint64_t)pl->grid ? (uint32_t)a.segCount : (uint32_t)pl->grid;
+    hipLaunchKernelGGL(sieve_segments, dim3(grid), dim3(kThreads), 0, as_stream(stream), a);
+    FFX_HIP(hipGetLastError());
+    return FFX_OK;
+}
+
+extern "C" {
+
+int ffx_sieve_plan_create(ffx_sieve_plan **out, uint64_t maxPrimeMapValue)
+{
+    const uint64_t sq = isqrt_u64(maxPrimeMapValue);
+    if (sq > 0xFFFFFFFFull) return fail(FFX_ERANGE, "sieve plan: maxPrimeMapValue too large");
+    std::vector<uint8_t> comp(sq + 1, 0);
+    std::vector<uint32_t> primes;
+    for (uint64_t p = 3; p <= sq; p += 2) {
+        if (comp[p]) continue;
+        primes.push_back((uint32_t)p);
+        for (uint64_t m = p * p; m <= sq; m += 2 * p) comp[m] = 1;
+    }
+    return plan_build(out, primes, maxPrimeMapValue);
+}
+
+int ffx_sieve_plan_create_from_primes(ffx_sieve_plan **out, const uint32_t *h_primes, uint32_t numPrimes,
+                                      uint64_t maxPrimeMapValue)
+{
+    if (!h_primes && numPrimes) return fail(FFX_EINVAL, "sieve plan: null prime list");
+    std::vector<uint32_t> primes;
+    uint32_t prev = 0;
+    for (uint32_t i = 0; i < numPrimes; ++i) {
+        const uint32_t p = h_primes[i];
+        if (p == 2) continue;
+        if (p < 3 || !(p & 1) || p <= prev) return fail(FFX_EINVAL, "sieve plan: prime list not odd ascending");
+        primes.push_back(p);
+        prev = p;
+    }
+    const uint64_t sq = isqrt_u64(maxPrimeMapValue);
+    if (sq >= 3 && (primes.empty() || primes.back() < sq)) {
+        // the list must reach isqrt(M): the largest odd prime <= sq must be in it
+        uint64_t q = sq | 1;
+        if (q > sq) q -= 2;
+        for (; q >= 3; q -= 2) {
+            bool isp = true;
+            for (uint64_t d = 3; d * d <= q; d += 2)
+                if (q % d == 0) { isp = false; break; }
+            if (isp) break;
+        }
+        if (q >= 3 && (primes.empty() || primes.back() < q))
+            return fail(FFX_EINVAL, "sieve plan: prime list stops below isqrt(maxPrimeMapValue)");
+    }
+    return plan_build(out, primes, maxPrimeMapValue);
+}
+
+int ffx_sieve_plan_destroy(ffx_sieve_plan *plan)
+{
+    if (!plan) return FFX_OK;
+    (void)hipFree(plan->dPrimes);
+    (void)hipFree(plan->dInv);
+    (void)hipFree(plan->dPatterns);
+    delete plan;
+    return FFX_OK;
+}
+
+int ffx_sieve_slab_async(const ffx_sieve_plan *plan, uint64_t segLo, uint64_t segHi, void *d_slab, void *stream)
+{
+    if (segLo & 15) return fail(FFX_EINVAL, "sieve slab: segLo %llu not a multiple of 16", (unsigned long long)segLo);
+    if (segHi <= segLo) return FFX_OK;
+    const uint64_t byteLo = segLo >> 4, byteHi = (segHi + 15) >> 4;
+    return launch(plan, static_cast<uint8_t *>(d_slab), byteLo, byteLo, byteHi, stream);
+}
+
+int ffx_sieve_map_async(const ffx_sieve_plan *plan, void *d_map, uint64_t mapBytes, void *stream)
+{
+    return launch(plan, static_cast<uint8_t *>(d_map), 0, 0, mapBytes, stream);
+}
+
+int ffx_sieve_map_range_async(const ffx_sieve_plan *plan, void *d_map, uint64_t byteLo, uint64_t byteHi,
+                              void *stream)
+{
+    return launch(plan, static_cast<uint8_t *>(d_map), 0, byteLo, byteHi, stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,149 @@
+// ff_hostcheck: host-only entry points of the ff_sieve host library for the
+// unit tests (no GPU): geometry, headers, partition, budget, config parsing,
+// line formatting and workload generation.  One subcommand per call; output
+// is "key value" lines or one record per line.
+#include <cinttypes>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "budget.h"
+#include "config.h"
+#include "emission.h"
+#include "geometry.h"
+#include "partition.h"
+#include "workload.h"
+
+using namespace ffg;
+
+static uint64_t u64(const char *s) { return std::strtoull(s, nullptr, 10); }
+
+int main(int argc, char **argv)
+{
+    if (argc < 2) {
+        std::fprintf(stderr, "usage: ff_hostcheck <cmd> ...\n");
+        return 2;
+    }
+    const std::string cmd = argv[1];
+    if (cmd == "leg" && argc >= 4) {
+        const LegGeometry g = computeLegGeometry(u64(argv[2]), u64(argv[3]));
+        std::printf("sumStart %" PRIu64 "\nsumLimit %" PRIu64 "\nmaxGeneratedPrime %" PRIu64 "\nproductLimit %" PRIu64
+                    "\nprimeLimit %" PRIu64 "\nmapBytes %" PRIu64 "\nmaxPrimeMapValue %" PRIu64 "\n",
+                    g.sumStart, g.sumLimit, g.maxGeneratedPrime, g.productLimit, g.primeLimit, g.mapBytes,
+                    g.maxPrimeMapValue);
+        return 0;
+    }
+    if (cmd == "seg" && argc >= 3) {
+        std::vector<std::string> pos(argv + 2, argv + argc);
+        uint64_t primeMem = kSegPrimeMem;
+        if (!pos.empty() && pos.back().rfind("primemem=", 0) == 0) {
+            primeMem = u64(pos.back().c_str() + 9);
+            pos.pop_back();
+        }
+        uint32_t a, b;
+        segPositionals(pos, &a, &b);
+        const SegGeometry g = computeSegGeometry(a, b, primeMem);
+        std::printf("sumStart %u\nsumLimit %u\nfatal %d\ncapped %d\nM %" PRIu64 "\nR %" PRIu64 "\nmapBytes %" PRIu64
+                    "\nsqrtLimit %" PRIu64 "\npmapSumLimit %" PRIu64 "\n",
+                    a, b, (int)g.fatal, (int)g.capped, g.maxPrimeMapValue, g.numPrimesRequested, g.mapBytes,
+                    g.sqrtLimit, g.fatal ? 0 : pmapSumLimit(g.maxPrimeMapValue));
+        for (const std::string &h : g.headers) std::printf("header %s\n", h.c_str());
+        return 0;
+    }
+    if (cmd == "chunks" && argc >= 4) {
+        const uint32_t a = (uint32_t)u64(argv[2]), b = (uint32_t)u64(argv[3]);
+        const std::vector<SumChunk> c =
+            argc >= 5 ? segRunChunks(a, b, u64(argv[4])) : segPartition(a, b);
+        for (const SumChunk &x : c) std::printf("%u %u\n", x.lo, x.hi);
+        return 0;
+    }
+    if (cmd == "budget" && argc >= 4) {
+        Config cfg;
+        for (int i = 4; i < argc; ++i) {
+            if (!std::strncmp(argv[i], "cap=", 4)) {
+                cfg.hasBudgetCap = true;
+                cfg.budgetCapBytes = u64(argv[i] + 4);
+            } else if (!std::strncmp(argv[i], "scratch=", 8)) {
+                cfg.hasScratch = true;
+                cfg.scratchBytes = u64(argv[i] + 8);
+            } else if (!std::strncmp(argv[i], "slab=", 5)) {
+                cfg.slabSizeBytes = u64(argv[i] + 5);
+            }
+        }
+        const bool shrink = argc >= 5 && !std::strcmp(argv[argc - 1], "shrink");
+        const double f = std::atof(argv[3]);
+        const DeviceBudget b = shrink ? shrinkFractionForAlloc(u64(argv[2]), cfg, f) : computeBudget(u64(argv[2]), cfg, f);
+        std::printf("fraction %.17g\nfreeUsed %" PRIu64 "\nbudget %" PRIu64 "\nscratch %" PRIu64 "\nheadroom %" PRIu64
+                    "\nbacking %" PRIu64 "\nslabCount %" PRIu64 "\n",
+                    b.fraction, b.freeUsed, b.budget, b.scratch, b.headroom, b.backing, b.slabCount);
+        return 0;
+    }
+    if (cmd == "size" && argc >= 3) {
+        uint64_t v = 0;
+        if (!parseSize(argv[2], &v)) return 1;
+        std::printf("%" PRIu64 "\n", v);
+        return 0;
+    }
+    if (cmd == "fraction" && argc >= 3) {
+        double v = 0;
+        if (!parseFraction(argv[2], &v)) return 1;
+        std::printf("%.17g\n", v);
+        return 0;
+    }
+    if (cmd == "devfrac" && argc >= 3) {
+        std::map<std::string, double> m;
+        if (!parseDeviceFractionSpec(argv[2], &m)) return 1;
+        for (const auto &kv : m) std::printf("%s %.17g\n", kv.first.c_str(), kv.second);
+        return 0;
+    }
+    if (cmd == "config") {
+        // parse env + argv[2..] like ff_sieve, then validate
+        Config cfg;
+        std::vector<std::string> pos;
+        std::vector<char *> av;
+        av.push_back(argv[0]);
+        for (int i = 2; i < argc; ++i) av.push_back(argv[i]);
+        if (loadEnv(&cfg) || parseArgs((int)av.size(), av.data(), &cfg, &pos) || validateConfig(&cfg)) return 1;
+        std::printf("fraction %.17g\nbudgetCap %" PRIu64 "\nscratch %" PRIu64 "\nslab %" PRIu64 "\nhostTier %" PRIu64
+                    "\nprimemem %" PRIu64 "\nmapCap %" PRIu64 "\npositionals %zu\n",
+                    cfg.globalFraction, cfg.hasBudgetCap ? cfg.budgetCapBytes : 0,
+                    cfg.hasScratch ? cfg.scratchBytes : 0, cfg.slabSizeBytes, cfg.hostTierCapBytes, cfg.primeMem,
+                    cfg.mapCap, pos.size());
+        return 0;
+    }
+    if (cmd == "human" && argc >= 3) {
+        std::printf("%s\n", bytesToHuman(u64(argv[2])).c_str());
+        return 0;
+    }
+    if (cmd == "workspace" && argc >= 3) {
+        std::printf("%" PRIu64 "\n", searchWorkspaceBytes(u64(argv[2])));
+        return 0;
+    }
+    if (cmd == "format" && argc >= 7) {
+        ffx_solution s;
+        s.sum = (uint32_t)u64(argv[3]);
+        s.low = (uint32_t)u64(argv[4]);
+        s.high = s.sum - s.low;
+        s.tags = (uint32_t)u64(argv[5]);
+        std::fputs(formatLine(u64(argv[2]), s).c_str(), stdout);
+        (void)argv[6];
+        return 0;
+    }
+    if (cmd == "batch" && argc >= 4) {
+        const std::vector<SumRange> r = batchRanges(u64(argv[2]), (uint32_t)u64(argv[3]),
+                                                    argc >= 5 ? (uint32_t)u64(argv[4]) : (1u << 22), 8.0, 16.0);
+        for (const SumRange &x : r) std::printf("%u %u\n", x.sumStart, x.sumLimit);
+        return 0;
+    }
+    if (cmd == "served" && argc >= 4) {
+        const double rate = argc >= 5 ? std::atof(argv[4]) : 1000.0;
+        const std::vector<Request> r =
+            servedRequests(u64(argv[2]), (uint32_t)u64(argv[3]), 1u << 18, 1u << 23, 6.0, 18.0, rate);
+        for (const Request &x : r) std::printf("%u %u %.9f\n", x.sumStart, x.sumLimit, x.arrivalS);
+        return 0;
+    }
+    std::fprintf(stderr, "ff_hostcheck: bad command\n");
+    return 2;
+}

@@ -1,0 +1,252 @@
+// ff_sieve <sumStart> <sumLimit> -- the Freudenthal "Sum and Product" search
+// on one MI355X, with ff_seg's stdout byte for byte.
+//
+// Host flow of source/main.cpp:144-573 with the GPU search made the default
+// (the reference hard-wired `useGpu = false`, :415): config -> geometry ->
+// device budget and run-gate -> header lines -> sieve on the device (the map
+// stays in HBM) -> search on the device -> ordered compaction -> emission in
+// ff_seg's chunk order -> timing lines.  STDOUT carries only ff_seg's bytes;
+// diagnostics go to STDERR with an [ff_sieve] prefix.
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "budget.h"
+#include "config.h"
+#include "emission.h"
+#include "ffx.h"
+#include "geometry.h"
+#include "partition.h"
+#include "scheduler.h"
+
+using namespace ffg;
+
+namespace {
+
+double msSince(std::chrono::steady_clock::time_point t)
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
+}
+
+void phase(const char *name, double ms) { std::fprintf(stderr, "ff_sieve timing: %s = %.3f ms\n", name, ms); }
+
+int err(const char *what, int rc)
+{
+    std::fprintf(stderr, "[ff_sieve] error: %s failed (%d): %s\n", what, rc, ffx_last_error());
+    return 1;
+}
+
+void printDevice(const ffx_device_info &d, int idx)
+{
+    std::fprintf(stderr,
+                 "[ff_sieve] device[%d] %s (%s vendor=%s bus=%s): free=%llu B (%s) total=%llu B (%s) CUs=%d "
+                 "lds/cu=%llu B l2=%llu B clock=%d kHz\n",
+                 idx, d.name, d.arch, d.vendor, d.busId, (unsigned long long)d.freeBytes,
+                 bytesToHuman(d.freeBytes).c_str(), (unsigned long long)d.totalBytes,
+                 bytesToHuman(d.totalBytes).c_str(), d.multiProcessorCount,
+                 (unsigned long long)d.sharedMemPerMultiprocessor, (unsigned long long)d.l2Bytes, d.clockKHz);
+}
+
+int listDevices()
+{
+    int n = 0;
+    int rc = ffx_device_count(&n);
+    if (rc) return err("device enumeration", rc);
+    if (n == 0) {
+        std::fprintf(stderr, "[ff_sieve] error: no devices enumerated\n");
+        return 1;
+    }
+    for (int i = 0; i < n; ++i) {
+        ffx_device_info d;
+        if ((rc = ffx_device_info_get(i, &d))) return err("device info", rc);
+        printDevice(d, i);
+    }
+    std::fprintf(stderr, "%d logical device(s)\n", n);
+    return 0;
+}
+
+// Device map size: the bitmap holds values up to mapM (ff_seg's M unless
+// --map-cap lowers it); Miller-Rabin answers above it.
+uint64_t deviceMapValue(const SegGeometry &g, uint64_t mapCap)
+{
+    if (!mapCap || mapCap >= g.maxPrimeMapValue) return g.maxPrimeMapValue;
+    return ((mapCap + 1) & ~15ull) - 1 > 15 ? ((mapCap + 1) & ~15ull) - 1 : 15;
+}
+
+int runLeg(const Config &cfg, const std::vector<std::string> &pos)
+{
+    uint32_t sumStart, sumLimit;
+    segPositionals(pos, &sumStart, &sumLimit);
+    const LegGeometry lg = computeLegGeometry(sumStart, sumLimit);
+    const SegGeometry sg = computeSegGeometry(sumStart, sumLimit, cfg.primeMem);
+    std::fprintf(stderr,
+                 "[ff_sieve] leg: sumStart=%u sumLimit=%u maxGeneratedPrime=%llu productLimit=%llu "
+                 "mapBytes=%llu B (%s) maxPrimeMapValue=%llu%s\n",
+                 sumStart, sumLimit, (unsigned long long)lg.maxGeneratedPrime, (unsigned long long)lg.productLimit,
+                 (unsigned long long)sg.mapBytes, bytesToHuman(sg.mapBytes).c_str(),
+                 (unsigned long long)sg.maxPrimeMapValue, sg.capped ? " (PRIMEMEM-capped)" : "");
+    if (sg.fatal) {
+        std::printf("%s\n", sg.headers[0].c_str());
+        std::printf("\nThe input parameters would indicate that more memory is required. Please adjust input "
+                    "parameters or Prime::maxPrimeAlloc.\n\n");
+        return 255;
+    }
+    if (sumLimit > FFX_MAX_SUM) {
+        std::fprintf(stderr, "[ff_sieve] error: sumLimit %u above the supported %u\n", sumLimit, FFX_MAX_SUM);
+        return 1;
+    }
+    if (!cfg.disableVendor.empty() && cfg.disableVendor == "amd") {
+        std::fprintf(stderr, "[ff_sieve] ERROR: FF_DISABLE_DEVICE=amd leaves no devices (enumerated: amd)\n");
+        return 1;
+    }
+    if (!cfg.deviceFilter.empty() && cfg.deviceFilter != "amd") {
+        std::fprintf(stderr, "[ff_sieve] ERROR: no device matches --devices=%s (enumerated: amd)\n",
+                     cfg.deviceFilter.c_str());
+        return 1;
+    }
+
+    auto tEnum = std::chrono::steady_clock::now();
+    int rc = ffx_init(cfg.device);
+    if (rc) return err("device init", rc);
+    ffx_device_info dev;
+    if ((rc = ffx_device_info_get(cfg.device, &dev))) return err("device info", rc);
+    phase("device enumeration", msSince(tEnum));
+    printDevice(dev, cfg.device);
+
+    // budget and run-gate (source/main.cpp:251-334)
+    double f = cfg.globalFraction;
+    auto it = cfg.deviceFractions.find("amd");
+    if (it == cfg.deviceFractions.end()) it = cfg.deviceFractions.find(std::to_string(cfg.device));
+    if (it != cfg.deviceFractions.end()) f = it->second;
+    const DeviceBudget b = computeBudget(dev.freeBytes, cfg, f);
+    std::fprintf(stderr,
+                 "[ff_sieve] device[%d] budget: f=%.3g budget=%llu B (%s) scratch=%llu B headroom=%llu B "
+                 "backing=%llu B (%s) slabs=%llu\n",
+                 cfg.device, b.fraction, (unsigned long long)b.budget, bytesToHuman(b.budget).c_str(),
+                 (unsigned long long)b.scratch, (unsigned long long)b.headroom, (unsigned long long)b.backing,
+                 bytesToHuman(b.backing).c_str(), (unsigned long long)b.slabCount);
+    if (cfg.slabSizeBytes > b.backing) {
+        std::fprintf(stderr, "[ff_sieve] validation error: --slab-size %llu B > backing %llu B\n",
+                     (unsigned long long)cfg.slabSizeBytes, (unsigned long long)b.backing);
+        return 1;
+    }
+    const uint64_t mapM = deviceMapValue(sg, cfg.mapCap);
+    const uint64_t mapBytes = (mapM >> 4) + 1;
+    const uint64_t capacity = b.backing + cfg.hostTierCapBytes;
+    std::fprintf(stderr, "[ff_sieve] aggregate: backing=%llu B hostTier=%llu B mapBytes=%llu B (%s) -> %s\n",
+                 (unsigned long long)b.backing, (unsigned long long)cfg.hostTierCapBytes,
+                 (unsigned long long)mapBytes, bytesToHuman(mapBytes).c_str(),
+                 capacity >= mapBytes && b.backing >= mapBytes ? "GATE PASS" : "GATE FAIL");
+    if (capacity < mapBytes || b.backing < mapBytes) {
+        std::fprintf(stderr,
+                     "[ff_sieve] ERROR: no device fits leg %u (map %llu B = %s): capacity %llu B\n"
+                     "[ff_sieve] remediation: raise FF_VRAM_BUDGET, or bound the device map with --map-cap "
+                     "(Miller-Rabin answers above it; stdout is unchanged)\n",
+                     sumLimit, (unsigned long long)mapBytes, bytesToHuman(mapBytes).c_str(),
+                     (unsigned long long)b.backing);
+        return 1;
+    }
+
+    for (const std::string &h : sg.headers) std::printf("%s\n", h.c_str());
+    std::fflush(stdout);
+
+    // ---- sieve
+    auto tSieve = std::chrono::steady_clock::now();
+    void *stream = ffx_stream_create();
+    if (!stream) return err("stream", FFX_EIO);
+    ffx_sieve_plan *sp = nullptr;
+    if ((rc = ffx_sieve_plan_create(&sp, mapM))) return err("sieve plan", rc);
+    void *dMap = ffx_alloc(((mapBytes + 255) & ~255ull));
+    if (!dMap) return err("map allocation", FFX_ENOMEM);
+    std::vector<uint8_t> hostMap;
+    if (!cfg.dumpMapFile.empty()) hostMap.resize(mapBytes);
+    SieveTiming st;
+    rc = sieveMap(sp, dMap, mapBytes, cfg.dumpMapFile.empty() ? 0 : cfg.slabSizeBytes,
+                  cfg.dumpMapFile.empty() ? nullptr : hostMap.data(), stream, &st);
+    if (rc) return err("sieve", rc);
+    const double sieveWall = msSince(tSieve);
+    std::fprintf(stderr, "[ff_sieve] sieve: %llu slab(s), kernel %.3f ms, %.1f GB/s\n", (unsigned long long)st.slabs,
+                 st.kernelMs, (double)mapBytes / (st.kernelMs * 1e6));
+    phase("sieve phase", sieveWall);
+
+    if (!cfg.dumpMapFile.empty()) {
+        FILE *fo = std::fopen(cfg.dumpMapFile.c_str(), "wb");
+        if (!fo || std::fwrite(hostMap.data(), 1, mapBytes, fo) != mapBytes) {
+            std::fprintf(stderr, "[ff_sieve] ERROR: cannot write %s\n", cfg.dumpMapFile.c_str());
+            if (fo) std::fclose(fo);
+            return 1;
+        }
+        std::fclose(fo);
+        std::fprintf(stderr, "[ff_sieve] dump-map: wrote %s (%llu bytes)\n", cfg.dumpMapFile.c_str(),
+                     (unsigned long long)mapBytes);
+        return 0;
+    }
+
+    // ---- search
+    auto tSearch = std::chrono::steady_clock::now();
+    const std::vector<SumChunk> chunks = segRunChunks(sumStart, sumLimit, sg.maxPrimeMapValue);
+    uint32_t lo = 0, hi = 0;
+    std::vector<ffx_solution> sols;
+    double searchKernelMs = 0;
+    if (chunkSpan(chunks, &lo, &hi)) {
+        if (hi > FFX_MAX_SUM) return err("search range", FFX_ERANGE);
+        ffx_search_plan *plan = nullptr;
+        if ((rc = ffx_search_plan_create(&plan, hi))) return err("search plan", rc);
+        const uint64_t nslots = (uint64_t)(hi - lo) / 2 + 1;
+        ffx_solution *dOut = (ffx_solution *)ffx_alloc(nslots * sizeof(ffx_solution));
+        uint32_t *dCount = (uint32_t *)ffx_alloc(4);
+        void *e0 = ffx_event_create(), *e1 = ffx_event_create();
+        if (!dOut || !dCount || !e0 || !e1) return err("search allocation", FFX_ENOMEM);
+        ffx_event_record(e0, stream);
+        if ((rc = ffx_search_async(plan, dMap, mapM, lo, hi, dOut, dCount, stream))) return err("search", rc);
+        ffx_event_record(e1, stream);
+        uint32_t count = 0;
+        if ((rc = ffx_copy_d2h_async(&count, dCount, 4, stream)) || (rc = ffx_stream_sync(stream)))
+            return err("search sync", rc);
+        ffx_event_elapsed_ms(e0, e1, &searchKernelMs);
+        sols.resize(count);
+        if (count && (rc = ffx_copy_d2h(sols.data(), dOut, (uint64_t)count * sizeof(ffx_solution))))
+            return err("solution copy", rc);
+        std::fprintf(stderr, "[ff_sieve] search: sums [%u, %u] (%llu odd), %u solutions, kernels %.3f ms, %.0f sums/s\n",
+                     lo, hi, (unsigned long long)nslots, count, searchKernelMs,
+                     (double)nslots / (searchKernelMs * 1e-3));
+        ffx_event_destroy(e0);
+        ffx_event_destroy(e1);
+        ffx_free(dOut);
+        ffx_free(dCount);
+        ffx_search_plan_destroy(plan);
+    }
+    uint64_t ordinal = 0;
+    emitChunks(stdout, chunks, sols.data(), sols.size(), &ordinal);
+    const double searchWall = msSince(tSearch);
+    emitTimes(stdout, (uint64_t)(sieveWall * 1000.0), (uint64_t)(searchWall * 1000.0));
+    std::fflush(stdout);
+    phase("search phase", searchWall);
+    ffx_free(dMap);
+    ffx_sieve_plan_destroy(sp);
+    ffx_stream_destroy(stream);
+    return 0;
+}
+
+}  // namespace
+
+int main(int argc, char **argv)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    int ret = 0;
+    {
+        Config cfg;
+        std::vector<std::string> pos;
+        if (loadEnv(&cfg) || parseArgs(argc, argv, &cfg, &pos) || validateConfig(&cfg)) ret = 1;
+        else if (cfg.listDevices) ret = listDevices();
+        else if (pos.empty()) {
+            std::fprintf(stderr, "usage: ff_sieve [options] [sumStart] sumLimit\n");
+            ret = listDevices();
+        } else ret = runLeg(cfg, pos);
+    }
+    phase("total", msSince(t0));
+    return ret;
+}
