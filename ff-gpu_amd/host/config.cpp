@@ -266,8 +266,8 @@ int validateConfig(Config *cfg)
                      (unsigned long long)cfg->slabSizeBytes);
         return -1;
     }
-    if (cfg->primeMem < 4096) {
-        std::fprintf(stderr, "[ff_sieve] validation error: --primemem must be >= 4096\n");
+    if (cfg->primeMem < 16) {
+        std::fprintf(stderr, "[ff_sieve] validation error: --primemem must be >= 16\n");
         return -1;
     }
     if (cfg->noHostTier) {

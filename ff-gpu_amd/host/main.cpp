@@ -242,10 +242,7 @@ int main(int argc, char **argv)
         std::vector<std::string> pos;
         if (loadEnv(&cfg) || parseArgs(argc, argv, &cfg, &pos) || validateConfig(&cfg)) ret = 1;
         else if (cfg.listDevices) ret = listDevices();
-        else if (pos.empty()) {
-            std::fprintf(stderr, "usage: ff_sieve [options] [sumStart] sumLimit\n");
-            ret = listDevices();
-        } else ret = runLeg(cfg, pos);
+        else ret = runLeg(cfg, pos);  // no positionals: ff_seg's default leg 5..2627
     }
     phase("total", msSince(t0));
     return ret;

@@ -39,6 +39,10 @@
  *   --counters         print reference-order work counters to stderr
  *   --search-sample K  search only every K-th odd sum (timing sample; the
  *                      output is then NOT ff_seg's)
+ *   --chunks           print ff_seg's chunk list ("lo hi" per line, print
+ *                      order) for the positional range and exit
+ *   --mr               read odd n > 17 from stdin, print "n 0|1" from the
+ *                      Miller-Rabin routine alone
  */
 #define _GNU_SOURCE
 #include <math.h>
@@ -749,7 +753,7 @@ int main(int argc, char **argv)
     int threads = (int)sysconf(_SC_NPROCESSORS_ONLN);
     uint64_t primemem = SEG_PRIMEMEM_DEFAULT, mapCap = 0;
     const char *dumpMap = NULL;
-    int counters = 0, solve = 0;
+    int counters = 0, solve = 0, chunksOnly = 0, mrOnly = 0;
     uint32_t sample = 0;
     long long pos[2];
     int npos = 0;
@@ -760,6 +764,8 @@ int main(int argc, char **argv)
         else if (!strcmp(argv[i], "--dump-map") && i + 1 < argc) dumpMap = argv[++i];
         else if (!strcmp(argv[i], "--counters")) counters = 1;
         else if (!strcmp(argv[i], "--solve")) solve = 1;
+        else if (!strcmp(argv[i], "--chunks")) chunksOnly = 1;
+        else if (!strcmp(argv[i], "--mr")) mrOnly = 1;
         else if (!strcmp(argv[i], "--search-sample") && i + 1 < argc) sample = (uint32_t)atoi(argv[++i]);
         else if (argv[i][0] == '-' && argv[i][1] == '-') {
             fprintf(stderr, "ffref: unknown option %s\n", argv[i]);
@@ -768,6 +774,11 @@ int main(int argc, char **argv)
     }
     if (threads < 1) threads = 1;
     if (solve) return run_solve(threads, mapCap);
+    if (mrOnly) {
+        unsigned long long n;
+        while (scanf("%llu", &n) == 1) printf("%llu %d\n", n, (n > 17 && (n & 1)) ? miller_rabin(n) : -1);
+        return 0;
+    }
     /* positional handling of ff_seg main (:844-855) */
     uint32_t sumStart, sumLimit;
     if (npos >= 2) {
@@ -776,6 +787,14 @@ int main(int argc, char **argv)
     } else {
         sumStart = 5;
         sumLimit = npos == 1 ? (uint32_t)(int)pos[0] : 2627;
+    }
+    if (chunksOnly) {
+        if (sumStart < 5) sumStart = 5;
+        if (sumStart > sumLimit) sumLimit = sumStart;
+        chunk_t ch[2 * SEG_PARTITION_THREADS];
+        int n = seg_partition(sumStart, sumLimit, ch);
+        for (int i = 0; i < n; ++i) printf("%u %u\n", ch[i].lo, ch[i].hi);
+        return 0;
     }
     return run_seg(sumStart, sumLimit, primemem, threads, dumpMap, counters, sample);
 }

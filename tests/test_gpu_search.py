@@ -1,0 +1,129 @@
+"""Search kernel parity (GPU) through the C ABI: golden solutions, the
+oracle's pure per-sum answers (also with a small map, so Miller-Rabin
+decides most probes), the reference-compatible record entry point and the
+ordered compaction."""
+import random
+import sys
+
+import numpy as np
+import pytest
+
+import ffxpy
+from conftest import golden_solutions, run_ffref, tags_to_flags
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def device():
+    ffxpy.init(0)
+
+
+@pytest.fixture(scope="module")
+def c1_map():
+    M = 268435471  # ff_seg geometry of 5..65535
+    return ffxpy.sieve_map(M), M
+
+
+def test_c1_solutions_match_goldens(c1_map):
+    dmap, M = c1_map
+    say("c1 search")
+    got = ffxpy.search(dmap, M, 5, 65535)
+    want = [(s, lo, tags_to_flags(t)) for s, lo, t in golden_solutions() if s <= 65535]
+    assert len(got) == len(want) == 2357
+    assert [(int(r["sum"]), int(r["low"]), int(r["tags"])) for r in got] == want
+    assert all(int(r["high"]) == int(r["sum"]) - int(r["low"]) for r in got)
+
+
+def test_subranges_and_tiny_ranges(c1_map):
+    dmap, M = c1_map
+    gold = {s: lo for s, lo, _ in golden_solutions() if s <= 65535}
+    for a, b in [(5, 5), (17, 17), (16, 18), (6, 6), (100, 99), (65001, 65535), (4001, 4061), (30001, 30030)]:
+        say("range", a, b)
+        got = ffxpy.search(dmap, M, a, b)
+        first = max(5, a | 1)
+        want = [(s, gold[s]) for s in range(first, b + 1, 2) if s in gold]
+        assert [(int(r["sum"]), int(r["low"])) for r in got] == want, (a, b)
+
+
+def _oracle_solve(sums, map_cap):
+    p = run_ffref(["--solve", "--map-cap", map_cap], stdin="\n".join(map(str, sums)).encode(), timeout=900)
+    return {int(s): int(lo) for s, lo, _ in (l.split() for l in p.stdout.decode().split("\n") if l)}
+
+
+@pytest.mark.parametrize("map_value", [2 ** 24 - 1, 2 ** 31 - 1])
+def test_slots_match_oracle_beyond_goldens(map_value):
+    # ranges up to 2^22 + 2^16 (the batched workload's reach) with a map far
+    # below ff_seg's: most probes are answered by Miller-Rabin
+    rng = random.Random(11 + map_value % 7)
+    ranges = [(a, a + 63) for a in (rng.randrange(2 ** 21, 2 ** 22 + 2 ** 16) | 1 for _ in range(6))]
+    ranges += [(5, 3001), (2 ** 20 - 63, 2 ** 20 + 63)]
+    top = max(b for _, b in ranges)
+    say("beyond goldens", map_value, ranges)
+    dmap = ffxpy.sieve_map(map_value)
+    plan = ffxpy.SearchPlan(top)
+    sums = []
+    got = {}
+    for a, b in ranges:
+        slots = ffxpy.search_slots(dmap, map_value, a, b, plan)
+        first = max(5, a | 1)
+        for i, v in enumerate(slots):
+            s = first + 2 * i
+            sums.append(s)
+            got[s] = int(v) & ffxpy.SLOT_LOW_MASK
+    want = _oracle_solve(sums, 2 ** 31 - 1)
+    assert got == want
+
+
+def test_records_entry_point(c1_map):
+    dmap, M = c1_map
+    lib = ffxpy.load()
+    n = (65535 - 5) // 2 + 1
+    rec = ffxpy.DeviceBuffer(n * 32)
+    cnt = ffxpy.DeviceBuffer(4)
+    ffxpy.check(lib.ffx_memset_async(rec.ptr, 0, n * 32, None))
+    ffxpy.check(lib.ffx_memset_async(cnt.ptr, 0, 4, None))
+    ffxpy.check(lib.ffx_device_sync())
+    ffxpy.check(lib.ffx_search_records(0, dmap.ptr, M, 5, 65535, cnt.ptr, rec.ptr), "ffx_search_records")
+    r = rec.to_host(n * 32).view(ffxpy.RECORD_DTYPE)
+    assert int(cnt.to_host(4, np.uint32)[0]) == 2357
+    gold = {s: lo for s, lo, _ in golden_solutions() if s <= 65535}
+    solved = r[r["sum"] != 0]
+    assert {int(x["sum"]): int(x["low"]) for x in solved} == gold
+    idx = np.nonzero(r["sum"])[0]
+    assert np.array_equal(r["sum"][idx], 5 + 2 * idx)  # sum-indexed slots
+
+
+def test_compaction_preserves_slot_order():
+    lib = ffxpy.load()
+    rng = np.random.default_rng(5)
+    for n in [1, 63, 4096, 4097, 100000, 3 * 4096 * 1024 + 17]:
+        say("compact", n)
+        slots = np.where(rng.random(n) < 0.07, rng.integers(1, 1 << 20, n), 0).astype(np.uint32)
+        d = ffxpy.DeviceBuffer(n * 4)
+        d.from_host(slots)
+        out = ffxpy.DeviceBuffer(n * 16)
+        cnt = ffxpy.DeviceBuffer(4)
+        plan = ffxpy.SearchPlan(1000)
+        ffxpy.check(lib.ffx_compact_async(plan.h, d.ptr, n, 1001, out.ptr, cnt.ptr, None))
+        ffxpy.check(lib.ffx_device_sync())
+        c = int(cnt.to_host(4, np.uint32)[0])
+        idx = np.nonzero(slots)[0]
+        assert c == len(idx)
+        got = out.to_host(c * 16).view(ffxpy.SOLUTION_DTYPE) if c else np.zeros(0, ffxpy.SOLUTION_DTYPE)
+        assert np.array_equal(got["sum"], (1001 + 2 * idx).astype(np.uint32))
+        assert np.array_equal(got["low"], slots[idx] & ffxpy.SLOT_LOW_MASK)
+
+
+def test_search_rejects_bad_arguments(c1_map):
+    dmap, M = c1_map
+    lib = ffxpy.load()
+    plan = ffxpy.SearchPlan(1000)
+    out = ffxpy.DeviceBuffer(64)
+    assert lib.ffx_search_slots_async(plan.h, dmap.ptr, M, 5, 5001, out.ptr, None) == -34  # above the plan
+    assert lib.ffx_search_slots_async(plan.h, dmap.ptr, 7, 5, 99, out.ptr, None) == -22    # map too small

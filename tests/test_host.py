@@ -1,0 +1,163 @@
+"""Host logic of ff_sieve without a GPU: geometry, headers, budget, config,
+ff_seg's chunk split, line format, workloads (ff-gpu_amd/bin/ff_hostcheck).
+Known-answer values follow test/source/budget_selftest.cpp:43-181."""
+import json
+import os
+import random
+import subprocess
+
+import pytest
+
+from conftest import GOLDEN, ROOT, golden_headers, golden_solutions, run_ffref, tags_to_flags
+
+HC = os.path.join(ROOT, "ff-gpu_amd", "bin", "ff_hostcheck")
+
+
+def hc(*args, env=None, check=True):
+    p = subprocess.run([HC] + [str(a) for a in args], capture_output=True, text=True, env=env)
+    if check:
+        assert p.returncode == 0, (args, p.stderr)
+    return p
+
+
+def kv(*args):
+    out = {}
+    for line in hc(*args).stdout.splitlines():
+        k, _, v = line.partition(" ")
+        out.setdefault(k, []).append(v)
+    return {k: (v[0] if len(v) == 1 else v) for k, v in out.items()}
+
+
+def test_leg_geometry_kats():
+    g = kv("leg", 5, 2097152)
+    assert g["maxGeneratedPrime"] == "1048576"
+    assert g["productLimit"] == "1099511627776"
+    assert g["primeLimit"] == "274877906946"
+    assert g["mapBytes"] == "17179869185"
+    assert g["maxPrimeMapValue"] == "274877906959"
+    assert kv("leg", 5, 1048576)["mapBytes"] == "4294967297"
+    assert kv("leg", 5, 65536)["mapBytes"] == "16777217"
+    assert kv("leg", 6, 100)["sumStart"] == "7"
+    assert hc("workspace", 2097152).stdout.strip() == "24960"
+
+
+def test_seg_headers_match_goldens():
+    for leg, (h1, h2) in golden_headers().items():
+        # sumLimit leg-1 has the same geometry as leg (same (L+1)>>1)
+        for L in (leg, leg - 1):
+            g = kv("seg", 5, L)
+            assert g["header"] == [h1, h2], L
+
+
+def test_seg_headers_match_ff_seg_cases():
+    for c in json.load(open(os.path.join(GOLDEN, "seg_cases.json"))):
+        args = list(c["args"]) + ([f"primemem={c['primemem']}"] if c["primemem"] else [])
+        g = kv("seg", *args) if c["args"] else kv("seg", "primemem=" + str(c["primemem"] or 91268055040))
+        want = [l for l in c["stdout"].split("\n") if l and ") sum" not in l and "time:" not in l
+                and "input parameters" not in l]
+        hdr = g["header"] if isinstance(g["header"], list) else [g["header"]]
+        if c["rc"] & 255:
+            assert g["fatal"] == "1" and hdr[0] == want[0]
+        else:
+            assert hdr == want, c["args"]
+
+
+def test_capped_geometry_2_23():
+    g = kv("seg", 5, 8388608)
+    assert g["capped"] == "1"
+    assert g["header"][1].startswith("numPrimesRequested ")
+    assert g["pmapSumLimit"] == "4833682"
+
+
+@pytest.mark.parametrize("rng_seed", [1, 2])
+def test_chunks_match_oracle(ffref, rng_seed):
+    # the host split equals the oracle's (which ff_seg's own stdout pins)
+    rng = random.Random(rng_seed)
+    cases = [(5, n) for n in range(5, 200)] + [(a, a + rng.randrange(0, 400)) for a in
+                                               (rng.randrange(5, 10 ** 6) for _ in range(100))]
+    for a, b in cases:
+        want = run_ffref(["--chunks", a, b]).stdout.decode()
+        assert hc("chunks", a | 1, b).stdout == want, (a, b)
+
+
+def test_budget_kats():
+    free = 34182397952
+    b = kv("budget", free, 0.90)
+    assert int(b["freeUsed"]) == int(0.90 * free)
+    assert int(b["scratch"]) == 1 << 30
+    assert int(b["backing"]) == (int(b["budget"]) - (1 << 30) - (64 << 20)) // 4096 * 4096
+    b = kv("budget", free, 0.90, "cap=21474836480")
+    assert int(b["budget"]) == 21474836480
+    assert int(b["backing"]) == 21474836480 - (1 << 30) - (64 << 20)
+    assert int(b["slabCount"]) == 18
+    b = kv("budget", free, 0.90, "cap=8589934592")
+    assert int(b["slabCount"]) == 6
+    assert 2 * int(b["backing"]) < 17179869185  # two 8 GiB caps do not hold the 2^21 map
+    # one shrink step toward the 0.50 floor
+    s1 = kv("budget", 21474836480, 0.90, "shrink")
+    assert abs(float(s1["fraction"]) - 0.70) < 1e-9
+    s2 = kv("budget", 21474836480, float(s1["fraction"]), "shrink")
+    assert abs(float(s2["fraction"]) - 0.60) < 1e-9
+    assert int(s2["backing"]) < int(s1["backing"])
+    fl = kv("budget", 21474836480, 0.50, "shrink")
+    assert float(fl["fraction"]) == 0.50
+
+
+def test_size_and_fraction_parsers():
+    for s, v in [("20GiB", 21474836480), ("8GiB", 8589934592), ("1GiB", 1 << 30), ("1024MiB", 1 << 30),
+                 ("64MiB", 67108864), ("4096", 4096), ("1.5GiB", 1610612736), ("0", 0)]:
+        assert hc("size", s).stdout.strip() == str(v)
+    for s in ["auto", "", "abc", "-5GiB", "20GiBX"]:
+        assert hc("size", s, check=False).returncode != 0
+    assert float(hc("fraction", "0.90").stdout) == 0.9
+    assert hc("fraction", "x", check=False).returncode != 0
+    assert hc("devfrac", "amd=0.001").stdout.split() == ["amd", "0.001"]
+    for bad in ["amd", "amd=0.001,", "=0.5"]:
+        assert hc("devfrac", bad, check=False).returncode != 0
+    assert hc("human", 17179869185).stdout.strip() == "16.00 GiB"
+    assert hc("human", 1 << 30).stdout.strip() == "1.00 GiB"
+
+
+def test_config_validation_and_env():
+    env = {k: v for k, v in os.environ.items() if not k.startswith("FF_")}
+    assert hc("config", env=env).returncode == 0
+    assert hc("config", "--vram-fraction", "2", env=env, check=False).returncode != 0
+    assert hc("config", "--device-vram-fraction", "amd=0.001", env=env, check=False).returncode != 0
+    assert hc("config", "--vram-budget", "0", env=env, check=False).returncode != 0
+    assert hc("config", "--slab-size", "3", env=env, check=False).returncode != 0
+    assert hc("config", "--bogus", env=env, check=False).returncode != 0
+    assert hc("config", "--no-gpu", env=env, check=False).returncode != 0
+    # CLI overrides env; malformed env is a hard error
+    e2 = dict(env, FF_VRAM_BUDGET="20GiB")
+    out = hc("config", env=e2).stdout
+    assert "budgetCap 21474836480" in out
+    out = hc("config", "--vram-budget", "8GiB", env=e2).stdout
+    assert "budgetCap 8589934592" in out
+    assert hc("config", env=dict(env, FF_VRAM_BUDGET="lots"), check=False).returncode != 0
+    assert hc("config", env=dict(env, FF_DISABLE_DEVICE="intel"), check=False).returncode != 0
+    assert hc("config", env=dict(env, FF_PULL_WEIGHTS="amd=0"), check=False).returncode != 0
+    assert "primemem 3000" in hc("config", "--primemem", "3000", env=env).stdout
+
+
+def test_line_format_matches_goldens():
+    # every line of the 65536 golden, rebuilt by the C++ formatter
+    rows = [r for r in golden_solutions() if r[0] <= 65536]
+    from ffxpy import format_line
+    for i, (s, lo, t) in enumerate(rows[:400], 1):
+        line = hc("format", i, s, lo, tags_to_flags(t), 0).stdout
+        assert line == format_line(i, s, lo, s - lo, tags_to_flags(t))
+
+
+def test_workloads_are_seeded_and_shaped():
+    a = hc("batch", 20261203, 4096).stdout.split("\n")
+    assert a == hc("batch", 20261203, 4096).stdout.split("\n")
+    rows = [tuple(map(int, l.split())) for l in a if l]
+    assert len(rows) == 4096
+    for s, l in rows:
+        assert 5 <= s < (1 << 22) and (1 << 8) <= l - s + 1 <= (1 << 16)
+    r = [l.split() for l in hc("served", 20261204, 2000).stdout.split("\n") if l]
+    t = [float(x[2]) for x in r]
+    assert t == sorted(t)
+    for a_, b_, _ in r:
+        a_, b_ = int(a_), int(b_)
+        assert (1 << 18) <= b_ <= (1 << 23) and a_ >= 5 and b_ - a_ + 1 <= (1 << 18)
