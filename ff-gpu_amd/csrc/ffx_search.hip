@@ -20,12 +20,18 @@
 // of a wavefront visit 64 splits per round and count with ballots.
 //
 // Per-sum work differs by five orders of magnitude (most sums stop after a
-// few rounds; emitted sums visit all ~s/4 splits), so waves pull sums from
-// one atomic counter, largest sums first.  Splits that pass both PTP probes
-// go to a per-wave LDS queue and are resolved 64 at a time, one per lane, so
-// the rare factor-pair enumerations do not serialize the probe rounds.
-// Terms d, o < s are factored with a smallest-prime-factor table, so no
-// 64-bit trial division runs on the device.
+// few hundred splits; emitted sums visit all ~s/4 splits), so waves pull sums
+// from one atomic counter, largest sums first.  Splits are visited k
+// descending as the reference does (it stops far earlier on non-solutions
+// than k ascending), flattened over all k so every lane is busy, two rounds
+// of 64 in flight per wave.  A probe value with a prime factor in 5..31 is
+// rejected by residue tables in LDS before it costs a random map read.
+// Splits that pass both PTP probes go to a per-wave LDS queue; while a sum is
+// young every queued candidate is resolved at once (early exit), later they
+// are resolved 64 at a time, one per lane, so the rare factor-pair
+// enumerations do not serialize the probe rounds.  Terms d, o < s are
+// factored with a smallest-prime-factor table, so no 64-bit trial division
+// runs on the device.
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -38,10 +44,24 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / 64;
-constexpr int kQueueCap = 128;
-constexpr uint32_t kGrab = 4;     // sums taken per atomic
-constexpr int kMaxPrimes = 12;    // distinct odd primes of q < 2^50
-constexpr uint32_t kDBits = 27;   // queue entry: d | k << 27
+constexpr int kQueueCap = 192;     // < 64 left after a drain + 128 from one two-round step
+constexpr uint32_t kGrab = 2;      // sums taken per atomic
+constexpr int kMaxPrimes = 12;     // distinct odd primes of q < 3^31 (q <= s^2/16, s < 2^26)
+constexpr uint32_t kDBits = 27;    // queue entry: d | k << 27
+constexpr int kMaxSegs = 32;
+// While a sum has visited fewer splits than this, every queued candidate is
+// resolved after each step (most sums stop within a few hundred splits);
+// beyond it candidates are resolved 64 at a time.
+constexpr uint32_t kEagerSplits = 4096;
+
+// Residue filter: a probe value with a prime factor in 5..31 (and larger
+// than 31) is composite without touching the map.  One bit table per
+// modulus, bit r set when gcd(r, m) == 1.
+constexpr uint32_t kM1 = 5 * 7 * 11 * 13;  // 5005
+constexpr uint32_t kM2 = 17 * 19 * 23;     // 7429
+constexpr uint32_t kM3 = 29 * 31;          // 899
+constexpr uint32_t kT1 = (kM1 + 31) / 32, kT2 = (kM2 + 31) / 32, kT3 = (kM3 + 31) / 32;
+constexpr uint32_t kTabWords = kT1 + kT2 + kT3;
 
 struct SearchArgs {
     const uint8_t *map;
@@ -52,6 +72,7 @@ struct SearchArgs {
     uint32_t nsums;           // odd sums first, first+2, ...
     uint32_t *slots;
     uint32_t *counter;
+    const uint32_t *coprime;  // kTabWords words: the three residue tables
 };
 
 // ---------------------------------------------------------------- primality
@@ -67,7 +88,7 @@ __device__ __forceinline__ uint64_t mont_mul(uint64_t a, uint64_t b, uint64_t n,
 
 // Deterministic Miller-Rabin with ff_seg's bases {2,3,5,7,11,13,17} and
 // base count by magnitude (AskMillerRabin, segmentedSieve.C:364-387).
-// n odd, 17 < n < 2^63.
+// n odd, 17 <= n < 2^63.
 __device__ __noinline__ bool mr_is_prime(uint64_t n)
 {
     uint64_t inv = n;  // n * n == 1 mod 8
@@ -107,10 +128,9 @@ __device__ __noinline__ bool mr_is_prime(uint64_t n)
     return true;
 }
 
-__device__ __forceinline__ bool map_bit(const uint8_t *map, uint64_t n)
-{
-    return (map[n >> 4] >> (7 - ((n >> 1) & 7))) & 1;
-}
+__device__ __forceinline__ bool byte_bit(uint32_t byte, uint64_t n) { return (byte >> (7 - ((n >> 1) & 7))) & 1; }
+
+__device__ __forceinline__ bool map_bit(const uint8_t *map, uint64_t n) { return byte_bit(map[n >> 4], n); }
 
 // IsPrime for odd n >= 1 (segmentedSieve.C:283-296).
 __device__ __forceinline__ bool is_prime_odd(const SearchArgs &a, uint64_t n)
@@ -134,11 +154,28 @@ __device__ __forceinline__ uint64_t isqrt53(uint64_t v)
     return r;
 }
 
+template <uint32_t m>
+__device__ __forceinline__ uint32_t mod64(uint64_t x)
+{
+    constexpr uint32_t c = (uint32_t)((1ull << 32) % m);
+    const uint32_t t = ((uint32_t)(x >> 32) % m) * c + (uint32_t)x % m;  // < m^2 + m < 2^32
+    return t % m;
+}
+
+__device__ __forceinline__ bool tab_bit(const uint32_t *t, uint32_t r) { return (t[r >> 5] >> (r & 31)) & 1; }
+
+// False when odd x > 31 has a prime factor in 5..31 (x is then composite).
+__device__ __forceinline__ bool residue_ok(const uint32_t *tab, uint64_t x)
+{
+    if (x <= 31) return true;
+    return tab_bit(tab, mod64<kM1>(x)) && tab_bit(tab + kT1, mod64<kM2>(x)) && tab_bit(tab + kT1 + kT2, mod64<kM3>(x));
+}
+
 // ---------------------------------------------------------------- AB1
 
-// Adds the factorization of odd n (< spfLimit) to (pr, ex, nf), merging
-// primes already present.
-__device__ __forceinline__ void add_factors(const SearchArgs &a, uint32_t n, uint32_t *pr, uint32_t *ex, int &nf)
+// Adds the factorization of odd n (< spfLimit) to the prime list pr[0..nf)
+// (LDS, this lane's row) and the 5-bit exponent fields of ex.
+__device__ __forceinline__ void factor_into(const SearchArgs &a, uint32_t n, uint32_t *pr, uint64_t &ex, int &nf)
 {
     while (n > 1) {
         uint32_t p = a.spf[n >> 1];
@@ -148,28 +185,21 @@ __device__ __forceinline__ void add_factors(const SearchArgs &a, uint32_t n, uin
             n /= p;
             ++e;
         } while (n % p == 0);
-        bool merged = false;
-#pragma unroll
-        for (int i = 0; i < kMaxPrimes; ++i)
-            if (i < nf && pr[i] == p) {
-                ex[i] += e;
-                merged = true;
-            }
-        if (!merged) {
-#pragma unroll
-            for (int i = 0; i < kMaxPrimes; ++i)
-                if (i == nf) {
-                    pr[i] = p;
-                    ex[i] = e;
-                }
+        int i = 0;
+        while (i < nf && pr[i] != p) ++i;
+        if (i == nf) {
+            pr[nf] = p;
             ++nf;
         }
+        ex += (uint64_t)e << (5 * i);
     }
 }
 
 // AllButOneProductOfTermPairsOfSumOfFactorPairsHasSingleFactorPair(k, d*o)
-// (segmentedSieve.C:424-477), with q = d*o factored from d and o.
-__device__ __noinline__ bool ab1(const SearchArgs &a, uint32_t k, uint32_t d, uint32_t o)
+// (segmentedSieve.C:424-477), with q = d*o factored from d and o.  Divisors
+// of q come from an odometer over exponent fields; the cofactor q/dv is an
+// exact double division (q < 2^53).
+__device__ __noinline__ bool ab1(const SearchArgs &a, uint32_t k, uint32_t d, uint32_t o, uint32_t *pr)
 {
     const uint64_t q = (uint64_t)d * o;
     const uint64_t P = q << k;
@@ -177,41 +207,30 @@ __device__ __noinline__ bool ab1(const SearchArgs &a, uint32_t k, uint32_t d, ui
     const uint64_t Lk = L >> k;
     int bad = 0;
     if ((1ull << k) <= L && !is_prime_odd(a, (1ull << k) + q - 2)) ++bad;
-    uint32_t pr[kMaxPrimes], ex[kMaxPrimes], e[kMaxPrimes];
-    uint64_t pp[kMaxPrimes + 1];
+    uint64_t ex = 0;
     int nf = 0;
-#pragma unroll
-    for (int i = 0; i < kMaxPrimes; ++i) {
-        pr[i] = 1;
-        ex[i] = 0;
-        e[i] = 0;
-        pp[i] = 1;
-    }
-    pp[kMaxPrimes] = 1;
-    add_factors(a, d, pr, ex, nf);
-    add_factors(a, o, pr, ex, nf);
+    factor_into(a, d, pr, ex, nf);
+    factor_into(a, o, pr, ex, nf);
     const double qd = (double)q;
-    // odometer over exponent vectors; pp[i] = prod_{j>=i} pr[j]^e[j]
+    uint64_t dig = 0, dv = 1;
     for (;;) {
-        int i = 0;
-        while (i < nf && e[i] == ex[i]) ++i;
-        if (i == nf) break;
-        uint64_t top = 1;
-#pragma unroll
-        for (int j = 0; j < kMaxPrimes; ++j)
-            if (j == i) {
-                e[j] += 1;
-                pp[j] *= pr[j];
-                top = pp[j];
+        int j = 0;
+        for (; j < nf; ++j) {
+            const uint32_t sh = 5 * j;
+            const uint32_t e = (uint32_t)(dig >> sh) & 31, x = (uint32_t)(ex >> sh) & 31;
+            const uint32_t p = pr[j];
+            if (e < x) {
+                dig += 1ull << sh;
+                dv *= p;
+                break;
             }
-#pragma unroll
-        for (int j = 0; j < kMaxPrimes; ++j)
-            if (j < i) {
-                e[j] = 0;
-                pp[j] = top;
-            }
-        const uint64_t dv = top;
-        const uint64_t c = (uint64_t)(qd / (double)dv);  // exact: dv | q, q < 2^53
+            uint64_t pe = 1;  // digit j wraps: divide p^x back out
+            for (uint32_t t = 0; t < x; ++t) pe *= p;
+            dv = (uint64_t)((double)dv / (double)pe);
+            dig &= ~(31ull << sh);
+        }
+        if (j == nf) break;
+        const uint64_t c = (uint64_t)(qd / (double)dv);
         if (dv <= L && !is_prime_odd(a, dv + (c << k) - 2))
             if (++bad > 1) return false;
         if (dv <= Lk && !is_prime_odd(a, (dv << k) + c - 2))
@@ -233,8 +252,8 @@ __device__ __forceinline__ void wave_sync_lds()
 
 // Resolves queue entries [from, from+nb) of this wave's queue, one per lane;
 // updates the wave's valid count and, on the first valid split, its terms.
-__device__ __forceinline__ void drain(const SearchArgs &a, const uint2 *queue, int from, int nb, int lane, int &cnt,
-                                      uint32_t &te, uint32_t &to)
+__device__ __forceinline__ void drain(const SearchArgs &a, const uint2 *queue, int from, int nb, int lane,
+                                      uint32_t *pr, int &cnt, uint32_t &te, uint32_t &to)
 {
     bool valid = false;
     uint32_t ee = 0, oo = 0;
@@ -243,7 +262,7 @@ __device__ __forceinline__ void drain(const SearchArgs &a, const uint2 *queue, i
         const uint32_t d = ent.x & ((1u << kDBits) - 1), k = ent.x >> kDBits;
         oo = ent.y;
         ee = d << k;
-        valid = ab1(a, k, d, oo);
+        valid = ab1(a, k, d, oo, pr);
     }
     const uint64_t vm = __ballot(valid);
     if (vm) {
@@ -257,7 +276,17 @@ __device__ __forceinline__ void drain(const SearchArgs &a, const uint2 *queue, i
     }
 }
 
-__device__ uint32_t solve_sum_wave(const SearchArgs &a, uint32_t s, int lane, uint2 *queue)
+// One split (k, d) of sum s through the PTP probes of DoesPeterKnow
+// (:632-639) and the family filter of Power2Odd/Power2Even (:578, :619).
+// Two splits per lane are carried through each stage together so their
+// random map reads are in flight at the same time.
+struct Split {
+    uint32_t k, d, o;
+    bool live;
+};
+
+__device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const uint32_t *tab, uint32_t s, int lane,
+                                                   uint2 *queue, uint2 *segs, uint32_t *pr)
 {
     // statement 1 (RunIt :806-807)
     if (is_prime_odd(a, s - 2)) return 0;
@@ -267,59 +296,131 @@ __device__ uint32_t solve_sum_wave(const SearchArgs &a, uint32_t s, int lane, ui
     uint32_t te = 0, to = 0;
     const uint64_t lt = lanes_below(lane);
 
-    // Power2Prime (:547-564): lane l tests k = l + 2
+    // Power2Prime (:547-564): lane l tests k = l + 2; composite odd terms
+    // become Power2Composite candidates (:591-605)
     {
         const uint32_t k = 2 + lane;
         const bool act = k < 32 && (1u << k) < s - 2;
-        const bool pr = act && is_prime_odd(a, s - (1u << k));
-        const uint64_t pm = __ballot(pr), cm = __ballot(act && !pr);
+        const bool isp = act && is_prime_odd(a, s - (1u << k));
+        const uint64_t pm = __ballot(isp), cm = __ballot(act && !isp);
         cnt = __popcll(pm);
         if (cnt > 1) return 0;
         if (cnt == 1) {
             te = 1u << (2 + __ffsll((unsigned long long)pm) - 1);
             to = s - te;
         }
-        // composite odd terms feed Power2Composite (:591-605) through the queue
-        if (act && !pr) queue[__popcll(cm & lt)] = make_uint2(1u | (k << kDBits), s - (1u << k));
+        if (act && !isp) queue[__popcll(cm & lt)] = make_uint2(1u | (k << kDBits), s - (1u << k));
         qn = __popcll(cm);
+    }
+
+    // Power2Odd and Power2Even splits, k descending as the reference visits
+    // them (:570-572, :611-613), both families interleaved by k, flattened
+    // into one index space: segment j is k = kTop - j, d = d0, d0+2, ...
+    const uint32_t t3 = (s - 3) / 3;
+    const int kTop = t3 >= 4 ? 31 - __clz(t3) : 0;  // largest k with 3*2^k <= s-3
+    const int nseg = kTop >= 2 ? kTop - 1 : 0;
+    uint32_t total = 0;
+    {
+        uint32_t n = 0, kj = 0, d0 = 0;
+        if (lane < nseg) {
+            kj = (uint32_t)(kTop - lane);
+            d0 = (kj & 1) ? 5 : 3;
+            const uint32_t D = (s - 3) >> kj;  // d*2^k < s-2  <=>  d <= (s-3)>>k
+            n = D >= d0 ? (D - d0) / 2 + 1 : 0;
+        }
+        uint32_t incl = n;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t v = __shfl_up(incl, off);
+            if (lane >= off) incl += v;
+        }
+        if (lane < nseg) segs[lane] = make_uint2(incl - n, kj | (d0 << 8));
+        total = nseg ? __shfl(incl, nseg - 1) : 0;
     }
     wave_sync_lds();
 
-    // Power2Odd / Power2Even (:566-589, :607-630), k ascending
-    for (uint32_t k = 2; k < 32; ++k) {
-        const uint32_t d0 = (k & 1) ? 5 : 3;
-        if (((uint64_t)d0 << k) > (uint64_t)(s - 3)) break;
-        const uint32_t D = (s - 3) >> k;  // d*2^k < s-2  <=>  d <= (s-3)>>k
-        for (uint32_t db = d0; db <= D; db += 128) {
-            const uint32_t d = db + 2 * lane;
-            bool cand = false;
-            uint32_t o = 0;
-            if (d <= D) {
-                o = s - (d << k);
-                const bool m3 = (d % 3 == 0) || (o % 3 == 0);
-                const bool pass = (k & 1) ? !m3 : (m3 || is_prime_odd(a, o));
-                if (pass) {
-                    const uint64_t q = (uint64_t)d * o;
-                    if (is_prime_odd(a, (1ull << k) + q - 2))
-                        cand = d == o || is_prime_odd(a, (uint64_t)d + ((uint64_t)o << k) - 2);
-                }
-            }
-            const uint64_t cm = __ballot(cand);
-            if (cm) {
-                if (cand) queue[qn + __popcll(cm & lt)] = make_uint2(d | (k << kDBits), o);
-                qn += __popcll(cm);
-                wave_sync_lds();
-                if (qn >= 64) {
-                    drain(a, queue, qn - 64, 64, lane, cnt, te, to);
-                    qn -= 64;
-                    if (cnt > 1) return 0;
-                }
-            }
+    int hint = 0;
+    for (uint32_t base = 0; base < total; base += 128) {
+        Split sp[2];
+        int segOf[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const uint32_t g = base + 64u * u + lane;
+            sp[u].live = g < total;
+            int j = hint;
+            if (sp[u].live)
+                while (j + 1 < nseg && g >= segs[j + 1].x) ++j;
+            segOf[u] = j;
+            const uint2 e = segs[j];
+            sp[u].k = e.y & 255;
+            sp[u].d = (e.y >> 8) + 2 * (g - e.x);
+            sp[u].o = sp[u].live ? s - (sp[u].d << sp[u].k) : 1;
+        }
+        hint = __shfl(segOf[1], 63);
+
+        // stage 1: 3 | d*o decides the family; Power2Even also takes prime o
+        bool st[2], needO[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const bool m3 = (sp[u].d % 3 == 0) || (sp[u].o % 3 == 0);
+            const bool odd = sp[u].k & 1;
+            needO[u] = sp[u].live && !odd && !m3;
+            st[u] = sp[u].live && (odd ? !m3 : true);
+        }
+        uint32_t ob[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) ob[u] = (needO[u] && sp[u].o <= a.M) ? a.map[sp[u].o >> 4] : 0u;
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+            if (needO[u]) st[u] = sp[u].o <= a.M ? byte_bit(ob[u], sp[u].o) : mr_is_prime(sp[u].o);
+
+        // stage 2: PTP(2^k + q), q = d*o
+        uint64_t x[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            x[u] = (1ull << sp[u].k) + (uint64_t)sp[u].d * sp[u].o - 2;
+            st[u] = st[u] && residue_ok(tab, x[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) ob[u] = (st[u] && x[u] <= a.M) ? a.map[x[u] >> 4] : 0u;
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+            if (st[u]) st[u] = x[u] <= a.M ? byte_bit(ob[u], x[u]) : mr_is_prime(x[u]);
+
+        // stage 3: PTP(d + o*2^k) unless d == o
+        bool need2[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            need2[u] = st[u] && sp[u].d != sp[u].o;
+            x[u] = (uint64_t)sp[u].d + ((uint64_t)sp[u].o << sp[u].k) - 2;
+            if (need2[u]) st[u] = residue_ok(tab, x[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) ob[u] = (need2[u] && st[u] && x[u] <= a.M) ? a.map[x[u] >> 4] : 0u;
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+            if (need2[u] && st[u]) st[u] = x[u] <= a.M ? byte_bit(ob[u], x[u]) : mr_is_prime(x[u]);
+
+        // survivors wait in the queue for AllButOne (:638)
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const uint64_t cm = __ballot(st[u]);
+            if (st[u]) queue[qn + __popcll(cm & lt)] = make_uint2(sp[u].d | (sp[u].k << kDBits), sp[u].o);
+            qn += __popcll(cm);
+        }
+        if (qn == 0) continue;
+        wave_sync_lds();
+        const bool eager = base + 128 <= kEagerSplits;
+        while (qn >= 64 || (eager && qn > 0)) {
+            const int nb = qn < 64 ? qn : 64;
+            drain(a, queue, qn - nb, nb, lane, pr, cnt, te, to);
+            qn -= nb;
+            if (cnt > 1) return 0;
         }
     }
+    wave_sync_lds();
     while (qn > 0) {
         const int nb = qn < 64 ? qn : 64;
-        drain(a, queue, qn - nb, nb, lane, cnt, te, to);
+        drain(a, queue, qn - nb, nb, lane, pr, cnt, te, to);
         qn -= nb;
         if (cnt > 1) return 0;
     }
@@ -341,11 +442,18 @@ __global__ void zero_u32(uint32_t *p, uint32_t n)
     if (i < n) p[i] = 0;
 }
 
-__global__ __launch_bounds__(kBlock) void search_sums(SearchArgs a)
+// 5 blocks of 4 waves per CU: the register budget that measured fastest
+// (more waves spill the AB1/Miller-Rabin paths, fewer leave HBM latency
+// exposed).
+__global__ __launch_bounds__(kBlock, 5) void search_sums(SearchArgs a)
 {
     __shared__ uint2 queues[kWavesPerBlock][kQueueCap];
-    const int lane = threadIdx.x & 63;
-    uint2 *queue = queues[threadIdx.x >> 6];
+    __shared__ uint2 segs[kWavesPerBlock][kMaxSegs];
+    __shared__ uint32_t primes[kBlock][kMaxPrimes];
+    __shared__ uint32_t tab[kTabWords];
+    for (uint32_t i = threadIdx.x; i < kTabWords; i += kBlock) tab[i] = a.coprime[i];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (;;) {
         uint32_t base = 0;
         if (lane == 0) base = atomicAdd(a.counter, kGrab);
@@ -355,7 +463,7 @@ __global__ __launch_bounds__(kBlock) void search_sums(SearchArgs a)
         for (uint32_t idx = base; idx < end; ++idx) {
             const uint32_t slotIdx = a.nsums - 1 - idx;  // largest sums first
             const uint32_t s = a.first + 2 * slotIdx;
-            const uint32_t res = solve_sum_wave(a, s, lane, queue);
+            const uint32_t res = solve_sum_wave(a, tab, s, lane, queues[w], segs[w], primes[threadIdx.x]);
             if (lane == 0) a.slots[slotIdx] = res;
         }
     }
@@ -465,6 +573,7 @@ __global__ void slots_to_records(const uint32_t *slots, uint64_t n, uint32_t fir
 
 struct ffx_search_plan {
     uint32_t sumLimitMax;
+    uint32_t *dCoprime;
     uint16_t *dSpf;
     uint32_t spfLimit;
     uint32_t *dCounter;
@@ -524,6 +633,21 @@ int ffx_search_plan_create(ffx_search_plan **out, uint32_t sumLimitMax)
     if (e == hipSuccess) e = hipMalloc(&pl->dSpf, spf.size() * 2);
     if (e == hipSuccess) e = hipMemcpy(pl->dSpf, spf.data(), spf.size() * 2, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMalloc(&pl->dCounter, 64);
+    // coprimality bit tables of the residue filter
+    std::vector<uint32_t> tab(kTabWords, 0);
+    const uint32_t mods[3] = {kM1, kM2, kM3}, offs[3] = {0, kT1, kT1 + kT2};
+    for (int t = 0; t < 3; ++t)
+        for (uint32_t r = 0; r < mods[t]; ++r) {
+            uint32_t g = r, m = mods[t];
+            while (m) {
+                const uint32_t tmp = g % m;
+                g = m;
+                m = tmp;
+            }
+            if (g == 1) tab[offs[t] + r / 32] |= 1u << (r % 32);
+        }
+    if (e == hipSuccess) e = hipMalloc(&pl->dCoprime, kTabWords * 4);
+    if (e == hipSuccess) e = hipMemcpy(pl->dCoprime, tab.data(), kTabWords * 4, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         int rc = hip_fail(e, "search plan upload");
         ffx_search_plan_destroy(pl);
@@ -538,6 +662,7 @@ int ffx_search_plan_destroy(ffx_search_plan *plan)
 {
     if (!plan) return FFX_OK;
     (void)hipFree(plan->dSpf);
+    (void)hipFree(plan->dCoprime);
     (void)hipFree(plan->dCounter);
     (void)hipFree(plan->dTile);
     (void)hipFree(plan->dSlots);
@@ -563,6 +688,7 @@ int ffx_search_slots_async(ffx_search_plan *plan, const void *d_map, uint64_t ma
     a.nsums = (sumLimit - first) / 2 + 1;
     a.slots = d_slots;
     a.counter = plan->dCounter;
+    a.coprime = plan->dCoprime;
     hipStream_t st = as_stream(stream);
     hipLaunchKernelGGL(zero_u32, dim3(1), dim3(64), 0, st, plan->dCounter, 16u);
     FFX_HIP(hipGetLastError());

@@ -154,8 +154,9 @@ int ffx_sieve_map_range_async(const ffx_sieve_plan *plan, void *d_map, uint64_t 
 #define FFX_TAG_LOW_POW2 0x20000000u
 #define FFX_TAG_HIGH_PRIME 0x40000000u
 #define FFX_TAG_HIGH_POW2 0x80000000u
-/* Largest sum the search accepts (terms must fit the slot encoding). */
-#define FFX_MAX_SUM 0x0FFFFFFFu
+/* Largest sum the search accepts: terms fit the slot encoding and every
+ * product q <= s^2/16 keeps prime exponents below 32. */
+#define FFX_MAX_SUM 0x03FFFFFFu
 
 /* One emitted line, in ascending sum order after compaction. */
 typedef struct ffx_solution {
