@@ -54,10 +54,10 @@ constexpr int kMaxSegs = 32;
 // beyond it candidates are resolved 64 at a time.
 constexpr uint32_t kEagerSplits = 4096;
 
-// Residue filter: a probe value with a prime factor in 5..31 (and larger
-// than 31) is composite without touching the map.  One bit table per
+// Residue filter: a probe value larger than 31 with a prime factor in 3..31
+// is composite without touching the map.  One bit table per
 // modulus, bit r set when gcd(r, m) == 1.
-constexpr uint32_t kM1 = 5 * 7 * 11 * 13;  // 5005
+constexpr uint32_t kM1 = 3 * 5 * 7 * 11 * 13;  // 15015
 constexpr uint32_t kM2 = 17 * 19 * 23;     // 7429
 constexpr uint32_t kM3 = 29 * 31;          // 899
 constexpr uint32_t kT1 = (kM1 + 31) / 32, kT2 = (kM2 + 31) / 32, kT3 = (kM3 + 31) / 32;
@@ -86,6 +86,17 @@ __device__ __forceinline__ uint64_t mont_mul(uint64_t a, uint64_t b, uint64_t n,
     return t >= n ? t - n : t;
 }
 
+// a*b mod n for a, b < n < 2^52: the quotient from doubles is off by at
+// most one, the remainder is exact in wrapping 64-bit arithmetic.
+__device__ __forceinline__ uint64_t mulmod52(uint64_t a, uint64_t b, uint64_t n)
+{
+    const uint64_t q = (uint64_t)((double)a * (double)b / (double)n);
+    int64_t r = (int64_t)(a * b - q * n);
+    while (r < 0) r += (int64_t)n;
+    while (r >= (int64_t)n) r -= (int64_t)n;
+    return (uint64_t)r;
+}
+
 // Deterministic Miller-Rabin with ff_seg's bases {2,3,5,7,11,13,17} and
 // base count by magnitude (AskMillerRabin, segmentedSieve.C:364-387).
 // n odd, 17 <= n < 2^63.
@@ -95,11 +106,18 @@ __device__ __noinline__ bool mr_is_prime(uint64_t n)
 #pragma unroll
     for (int i = 0; i < 5; ++i) inv *= 2 - n * inv;
     const uint64_t nneg = 0 - inv;
-    const uint64_t one = (0 - n) % n;  // 2^64 mod n
-    uint64_t r2 = one;
-    for (int i = 0; i < 64; ++i) {
-        r2 <<= 1;
-        if (r2 >= n) r2 -= n;
+    uint64_t one, r2;  // 2^64 mod n, 2^128 mod n
+    if (n < (1ull << 52)) {
+        const uint64_t t = n > (1ull << 32) ? (1ull << 32) : (1ull << 32) % n;
+        one = mulmod52(t, t, n);
+        r2 = mulmod52(one, one, n);
+    } else {
+        one = (0 - n) % n;
+        r2 = one;
+        for (int i = 0; i < 64; ++i) {
+            r2 <<= 1;
+            if (r2 >= n) r2 -= n;
+        }
     }
     const uint64_t mone = n - one;
     const uint32_t r = (uint32_t)__builtin_ctzll(n - 1);
@@ -130,7 +148,13 @@ __device__ __noinline__ bool mr_is_prime(uint64_t n)
 
 __device__ __forceinline__ bool byte_bit(uint32_t byte, uint64_t n) { return (byte >> (7 - ((n >> 1) & 7))) & 1; }
 
-__device__ __forceinline__ bool map_bit(const uint8_t *map, uint64_t n) { return byte_bit(map[n >> 4], n); }
+// Map bytes through a global-address-space pointer: global_load, not flat.
+__device__ __forceinline__ uint32_t map_byte(const uint8_t *map, uint64_t n)
+{
+    return ((const __attribute__((address_space(1))) uint8_t *)map)[n >> 4];
+}
+
+__device__ __forceinline__ bool map_bit(const uint8_t *map, uint64_t n) { return byte_bit(map_byte(map, n), n); }
 
 // IsPrime for odd n >= 1 (segmentedSieve.C:283-296).
 __device__ __forceinline__ bool is_prime_odd(const SearchArgs &a, uint64_t n)
@@ -164,7 +188,7 @@ __device__ __forceinline__ uint32_t mod64(uint64_t x)
 
 __device__ __forceinline__ bool tab_bit(const uint32_t *t, uint32_t r) { return (t[r >> 5] >> (r & 31)) & 1; }
 
-// False when odd x > 31 has a prime factor in 5..31 (x is then composite).
+// False when odd x > 31 has a prime factor in 3..31 (x is then composite).
 __device__ __forceinline__ bool residue_ok(const uint32_t *tab, uint64_t x)
 {
     if (x <= 31) return true;
@@ -174,17 +198,33 @@ __device__ __forceinline__ bool residue_ok(const uint32_t *tab, uint64_t x)
 // ---------------------------------------------------------------- AB1
 
 // Adds the factorization of odd n (< spfLimit) to the prime list pr[0..nf)
-// (LDS, this lane's row) and the 5-bit exponent fields of ex.
+// (LDS, this lane's row) and the 5-bit exponent fields of ex.  Quotients by
+// the factor p are exact, so they are products with p^-1 mod 2^32, and
+// p | n  <=>  n * p^-1 mod 2^32 <= (2^32-1)/p.
 __device__ __forceinline__ void factor_into(const SearchArgs &a, uint32_t n, uint32_t *pr, uint64_t &ex, int &nf)
 {
     while (n > 1) {
-        uint32_t p = a.spf[n >> 1];
-        if (p == 0) p = n;
+        uint32_t p;
+        if (n < a.spfLimit) {
+            p = a.spf[n >> 1];
+            if (p == 0) p = n;
+        } else {  // beyond the table (never for terms of a sum the plan covers)
+            p = n;
+            for (uint32_t f = 3; f * f <= n; f += 2)
+                if (n % f == 0) {
+                    p = f;
+                    break;
+                }
+        }
+        uint32_t inv = p;  // p * p == 1 mod 8
+#pragma unroll
+        for (int i = 0; i < 4; ++i) inv *= 2u - p * inv;
+        const uint32_t lim = 0xFFFFFFFFu / p;
         uint32_t e = 0;
         do {
-            n /= p;
+            n *= inv;
             ++e;
-        } while (n % p == 0);
+        } while (n * inv <= lim);
         int i = 0;
         while (i < nf && pr[i] != p) ++i;
         if (i == nf) {
@@ -196,17 +236,38 @@ __device__ __forceinline__ void factor_into(const SearchArgs &a, uint32_t n, uin
 }
 
 // AllButOneProductOfTermPairsOfSumOfFactorPairsHasSingleFactorPair(k, d*o)
-// (segmentedSieve.C:424-477), with q = d*o factored from d and o.  Divisors
-// of q come from an odometer over exponent fields; the cofactor q/dv is an
-// exact double division (q < 2^53).
+// (segmentedSieve.C:424-477) for the split s = d*2^k + o.  The split's own
+// factor pair (d*2^k, o) is always among the pairs counted and always has
+// PTP(s) false -- that is statement 1, which every searched sum satisfies --
+// so the reference's "exactly one" is "no other pair has PTP false": the
+// enumeration stops at the first other one.  q = d*o is factored from d and
+// o; divisors come from an odometer over exponent fields; the cofactor q/dv
+// is an exact double division (q < 2^53).  The f = 2^k pair is PTP(2^k + q),
+// already seen true by the caller for d > 1, and is the own pair for d = 1.
+// Terms beyond the map wait for Miller-Rabin until the map terms are done.
 __device__ __noinline__ bool ab1(const SearchArgs &a, uint32_t k, uint32_t d, uint32_t o, uint32_t *pr)
 {
     const uint64_t q = (uint64_t)d * o;
+    const uint64_t own = ((uint64_t)d << k) + o - 2;  // s - 2
     const uint64_t P = q << k;
     const uint64_t L = isqrt53(P);
     const uint64_t Lk = L >> k;
-    int bad = 0;
-    if ((1ull << k) <= L && !is_prime_odd(a, (1ull << k) + q - 2)) ++bad;
+    uint64_t late0 = 0, late1 = 0;
+#define FFX_TERM(X)                                     \
+    do {                                                \
+        const uint64_t x_ = (X);                        \
+        if (x_ != own) {                                \
+            if (x_ <= a.M) {                            \
+                if (!map_bit(a.map, x_)) return false;  \
+            } else if (!late0) {                        \
+                late0 = x_;                             \
+            } else if (!late1) {                        \
+                late1 = x_;                             \
+            } else if (!mr_is_prime(x_)) {              \
+                return false;                           \
+            }                                           \
+        }                                               \
+    } while (0)
     uint64_t ex = 0;
     int nf = 0;
     factor_into(a, d, pr, ex, nf);
@@ -231,12 +292,13 @@ __device__ __noinline__ bool ab1(const SearchArgs &a, uint32_t k, uint32_t d, ui
         }
         if (j == nf) break;
         const uint64_t c = (uint64_t)(qd / (double)dv);
-        if (dv <= L && !is_prime_odd(a, dv + (c << k) - 2))
-            if (++bad > 1) return false;
-        if (dv <= Lk && !is_prime_odd(a, (dv << k) + c - 2))
-            if (++bad > 1) return false;
+        if (dv <= L) FFX_TERM(dv + (c << k) - 2);
+        if (dv <= Lk) FFX_TERM((dv << k) + c - 2);
     }
-    return bad == 1;
+#undef FFX_TERM
+    if (late0 && !mr_is_prime(late0)) return false;
+    if (late1 && !mr_is_prime(late1)) return false;
+    return true;
 }
 
 // ---------------------------------------------------------------- per sum
@@ -368,7 +430,7 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
         }
         uint32_t ob[2];
 #pragma unroll
-        for (int u = 0; u < 2; ++u) ob[u] = (needO[u] && sp[u].o <= a.M) ? a.map[sp[u].o >> 4] : 0u;
+        for (int u = 0; u < 2; ++u) ob[u] = (needO[u] && sp[u].o <= a.M) ? map_byte(a.map, sp[u].o) : 0u;
 #pragma unroll
         for (int u = 0; u < 2; ++u)
             if (needO[u]) st[u] = sp[u].o <= a.M ? byte_bit(ob[u], sp[u].o) : mr_is_prime(sp[u].o);
@@ -381,7 +443,7 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
             st[u] = st[u] && residue_ok(tab, x[u]);
         }
 #pragma unroll
-        for (int u = 0; u < 2; ++u) ob[u] = (st[u] && x[u] <= a.M) ? a.map[x[u] >> 4] : 0u;
+        for (int u = 0; u < 2; ++u) ob[u] = (st[u] && x[u] <= a.M) ? map_byte(a.map, x[u]) : 0u;
 #pragma unroll
         for (int u = 0; u < 2; ++u)
             if (st[u]) st[u] = x[u] <= a.M ? byte_bit(ob[u], x[u]) : mr_is_prime(x[u]);
@@ -395,7 +457,7 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
             if (need2[u]) st[u] = residue_ok(tab, x[u]);
         }
 #pragma unroll
-        for (int u = 0; u < 2; ++u) ob[u] = (need2[u] && st[u] && x[u] <= a.M) ? a.map[x[u] >> 4] : 0u;
+        for (int u = 0; u < 2; ++u) ob[u] = (need2[u] && st[u] && x[u] <= a.M) ? map_byte(a.map, x[u]) : 0u;
 #pragma unroll
         for (int u = 0; u < 2; ++u)
             if (need2[u] && st[u]) st[u] = x[u] <= a.M ? byte_bit(ob[u], x[u]) : mr_is_prime(x[u]);
