@@ -220,6 +220,14 @@ int ffx_event_wait(void *ev, void *stream)
     return FFX_OK;
 }
 
+int ffx_event_query(void *ev)
+{
+    const hipError_t e = hipEventQuery(reinterpret_cast<hipEvent_t>(ev));
+    if (e == hipSuccess) return 0;
+    if (e == hipErrorNotReady) return 1;
+    return hip_fail(e, "hipEventQuery");
+}
+
 int ffx_event_sync(void *ev)
 {
     FFX_HIP(hipEventSynchronize(reinterpret_cast<hipEvent_t>(ev)));
@@ -237,6 +245,55 @@ int ffx_event_elapsed_ms(void *start, void *end, double *ms)
 int ffx_device_sync(void)
 {
     FFX_HIP(hipDeviceSynchronize());
+    return FFX_OK;
+}
+
+}  // extern "C"
+
+struct ffx_graph {
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+};
+
+extern "C" {
+
+int ffx_graph_begin(void *stream)
+{
+    if (!stream) return fail(FFX_EINVAL, "graph capture needs a created stream");
+    FFX_HIP(hipStreamBeginCapture(as_stream(stream), hipStreamCaptureModeThreadLocal));
+    return FFX_OK;
+}
+
+int ffx_graph_end(void *stream, ffx_graph **out)
+{
+    if (!out) return fail(FFX_EINVAL, "graph: null out");
+    *out = nullptr;
+    hipGraph_t g = nullptr;
+    FFX_HIP(hipStreamEndCapture(as_stream(stream), &g));
+    hipGraphExec_t ex = nullptr;
+    hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return hip_fail(e, "hipGraphInstantiate");
+    }
+    ffx_graph *r = new ffx_graph{g, ex};
+    *out = r;
+    return FFX_OK;
+}
+
+int ffx_graph_launch(ffx_graph *g, void *stream)
+{
+    if (!g) return fail(FFX_EINVAL, "graph: null");
+    FFX_HIP(hipGraphLaunch(g->exec, as_stream(stream)));
+    return FFX_OK;
+}
+
+int ffx_graph_destroy(ffx_graph *g)
+{
+    if (!g) return FFX_OK;
+    (void)hipGraphExecDestroy(g->exec);
+    (void)hipGraphDestroy(g->graph);
+    delete g;
     return FFX_OK;
 }
 

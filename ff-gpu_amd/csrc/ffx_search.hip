@@ -45,14 +45,15 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / 64;
 constexpr int kQueueCap = 192;     // < 64 left after a drain + 128 from one two-round step
-constexpr uint32_t kGrab = 2;      // sums taken per atomic
+constexpr uint32_t kGrab = 2;      // default sums taken per atomic
 constexpr int kMaxPrimes = 12;     // distinct odd primes of q < 3^31 (q <= s^2/16, s < 2^26)
 constexpr uint32_t kDBits = 27;    // queue entry: d | k << 27
 constexpr int kMaxSegs = 32;
-// While a sum has visited fewer splits than this, every queued candidate is
-// resolved after each step (most sums stop within a few hundred splits);
-// beyond it candidates are resolved 64 at a time.
+// Default: while a sum has visited fewer splits than this, every queued
+// candidate is resolved after each step (most sums stop within a few hundred
+// splits); beyond it candidates are resolved 64 at a time.
 constexpr uint32_t kEagerSplits = 4096;
+constexpr uint32_t kBlocksPerCU = 8;  // default persistent grid: CUs x this
 
 // Residue filter: a probe value larger than 31 with a prime factor in 3..31
 // is composite without touching the map.  One bit table per
@@ -68,12 +69,33 @@ struct SearchArgs {
     uint64_t M;               // bitmap answers odd n <= M
     const uint16_t *spf;      // smallest prime factor of odd n (index n>>1), 0 if n prime or 1
     uint32_t spfLimit;        // spf covers odd n < spfLimit
-    uint32_t first;           // first odd sum
-    uint32_t nsums;           // odd sums first, first+2, ...
+    uint32_t first;           // first odd sum (single range)
+    uint32_t nsums;           // slots to fill: odd sums first, first+2, ... or all intervals
     uint32_t *slots;
     uint32_t *counter;
     const uint32_t *coprime;  // kTabWords words: the three residue tables
+    const ffx_interval *iv;   // when niv > 0: slot g is sum iv[i].first + 2(g - iv[i].slotOffset)
+    uint32_t niv;
+    uint32_t grab;            // sums taken per atomic
+    uint32_t eager;           // splits visited before queued candidates wait for full batches
+    uint32_t bySum;           // slot of sum s is slots[(s - sumBase) / 2] instead of slot g
+    uint32_t sumBase;
+    uint64_t slotCap;         // bySum: slots outside [0, slotCap) are not written
 };
+
+// Sum of slot g: one range, or the interval holding g (binary search; the
+// wave's lanes agree on g, so the loads are uniform).
+__device__ __forceinline__ uint32_t sum_of_slot(const SearchArgs &a, uint32_t g)
+{
+    if (a.niv == 0) return a.first + 2 * g;
+    uint32_t lo = 0, hi = a.niv - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (a.iv[mid].slotOffset <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return a.iv[lo].first + 2 * (uint32_t)(g - a.iv[lo].slotOffset);
+}
 
 // ---------------------------------------------------------------- primality
 
@@ -471,7 +493,7 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
         }
         if (qn == 0) continue;
         wave_sync_lds();
-        const bool eager = base + 128 <= kEagerSplits;
+        const bool eager = base + 128 <= a.eager;
         while (qn >= 64 || (eager && qn > 0)) {
             const int nb = qn < 64 ? qn : 64;
             drain(a, queue, qn - nb, nb, lane, pr, cnt, te, to);
@@ -518,15 +540,18 @@ __global__ __launch_bounds__(kBlock, 5) void search_sums(SearchArgs a)
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (;;) {
         uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(a.counter, kGrab);
+        if (lane == 0) base = atomicAdd(a.counter, a.grab);
         base = __shfl(base, 0);
         if (base >= a.nsums) break;
-        const uint32_t end = base + kGrab < a.nsums ? base + kGrab : a.nsums;
+        const uint32_t end = base + a.grab < a.nsums ? base + a.grab : a.nsums;
         for (uint32_t idx = base; idx < end; ++idx) {
             const uint32_t slotIdx = a.nsums - 1 - idx;  // largest sums first
-            const uint32_t s = a.first + 2 * slotIdx;
+            const uint32_t s = sum_of_slot(a, slotIdx);
             const uint32_t res = solve_sum_wave(a, tab, s, lane, queues[w], segs[w], primes[threadIdx.x]);
-            if (lane == 0) a.slots[slotIdx] = res;
+            if (lane == 0) {
+                if (!a.bySum) a.slots[slotIdx] = res;
+                else if (s >= a.sumBase && ((s - a.sumBase) >> 1) < a.slotCap) a.slots[(s - a.sumBase) >> 1] = res;
+            }
         }
     }
 }
@@ -635,6 +660,7 @@ __global__ void slots_to_records(const uint32_t *slots, uint64_t n, uint32_t fir
 
 struct ffx_search_plan {
     uint32_t sumLimitMax;
+    uint32_t grab, eager, blocksPerCU, cus;
     uint32_t *dCoprime;
     uint16_t *dSpf;
     uint32_t spfLimit;
@@ -715,7 +741,11 @@ int ffx_search_plan_create(ffx_search_plan **out, uint32_t sumLimitMax)
         ffx_search_plan_destroy(pl);
         return rc;
     }
-    pl->grid = (cus > 0 ? cus : 256) * 8;  // 32 waves per CU
+    pl->cus = cus > 0 ? (uint32_t)cus : 256;
+    pl->grab = kGrab;
+    pl->eager = kEagerSplits;
+    pl->blocksPerCU = kBlocksPerCU;
+    pl->grid = (int)(pl->cus * pl->blocksPerCU);
     *out = pl;
     return FFX_OK;
 }
@@ -732,6 +762,24 @@ int ffx_search_plan_destroy(ffx_search_plan *plan)
     return FFX_OK;
 }
 
+static int launch_search(ffx_search_plan *plan, SearchArgs &a, void *stream)
+{
+    a.spf = plan->dSpf;
+    a.spfLimit = plan->spfLimit;
+    a.counter = plan->dCounter;
+    a.coprime = plan->dCoprime;
+    a.grab = plan->grab;
+    a.eager = plan->eager;
+    hipStream_t st = as_stream(stream);
+    hipLaunchKernelGGL(zero_u32, dim3(1), dim3(64), 0, st, plan->dCounter, 16u);
+    FFX_HIP(hipGetLastError());
+    const uint32_t need = (a.nsums + kWavesPerBlock - 1) / kWavesPerBlock;
+    const uint32_t grid = need < (uint32_t)plan->grid ? need : (uint32_t)plan->grid;
+    hipLaunchKernelGGL(search_sums, dim3(grid), dim3(kBlock), 0, st, a);
+    FFX_HIP(hipGetLastError());
+    return FFX_OK;
+}
+
 int ffx_search_slots_async(ffx_search_plan *plan, const void *d_map, uint64_t maxPrimeMapValue, uint32_t sumStart,
                            uint32_t sumLimit, uint32_t *d_slots, void *stream)
 {
@@ -741,24 +789,90 @@ int ffx_search_slots_async(ffx_search_plan *plan, const void *d_map, uint64_t ma
     if (sumLimit > plan->sumLimitMax)
         return fail(FFX_ERANGE, "search: sumLimit %u above the plan's %u", sumLimit, plan->sumLimitMax);
     if (maxPrimeMapValue < 15) return fail(FFX_EINVAL, "search: map too small");
-    SearchArgs a;
+    SearchArgs a = {};
     a.map = static_cast<const uint8_t *>(d_map);
     a.M = maxPrimeMapValue;
-    a.spf = plan->dSpf;
-    a.spfLimit = plan->spfLimit;
     a.first = first;
     a.nsums = (sumLimit - first) / 2 + 1;
     a.slots = d_slots;
-    a.counter = plan->dCounter;
-    a.coprime = plan->dCoprime;
-    hipStream_t st = as_stream(stream);
-    hipLaunchKernelGGL(zero_u32, dim3(1), dim3(64), 0, st, plan->dCounter, 16u);
-    FFX_HIP(hipGetLastError());
-    const uint32_t need = (a.nsums + kWavesPerBlock - 1) / kWavesPerBlock;
-    const uint32_t grid = need < (uint32_t)plan->grid ? need : (uint32_t)plan->grid;
-    hipLaunchKernelGGL(search_sums, dim3(grid), dim3(kBlock), 0, st, a);
-    FFX_HIP(hipGetLastError());
-    return FFX_OK;
+    return launch_search(plan, a, stream);
+}
+
+int ffx_search_intervals_async(ffx_search_plan *plan, const void *d_map, uint64_t maxPrimeMapValue,
+                               const ffx_interval *d_intervals, uint32_t nintervals, uint64_t totalSlots,
+                               uint32_t lastSum, uint32_t *d_slots, void *stream)
+{
+    if (!plan || !d_map || !d_slots || (!d_intervals && nintervals)) return fail(FFX_EINVAL, "search: null argument");
+    if (nintervals == 0 || totalSlots == 0) return FFX_OK;
+    if (lastSum > plan->sumLimitMax)
+        return fail(FFX_ERANGE, "search: interval sum %u above the plan's %u", lastSum, plan->sumLimitMax);
+    if (totalSlots > 0xFFFFFFFFull) return fail(FFX_ERANGE, "search: more than 2^32 slots");
+    if (maxPrimeMapValue < 15) return fail(FFX_EINVAL, "search: map too small");
+    SearchArgs a = {};
+    a.map = static_cast<const uint8_t *>(d_map);
+    a.M = maxPrimeMapValue;
+    a.nsums = (uint32_t)totalSlots;
+    a.slots = d_slots;
+    a.iv = d_intervals;
+    a.niv = nintervals;
+    return launch_search(plan, a, stream);
+}
+
+int ffx_search_intervals_by_sum_async(ffx_search_plan *plan, const void *d_map, uint64_t maxPrimeMapValue,
+                                      const ffx_interval *d_intervals, uint32_t nintervals, uint64_t totalSlots,
+                                      uint32_t lastSum, uint32_t slotSumBase, uint32_t *d_slots, uint64_t slotCapacity,
+                                      void *stream)
+{
+    if (!plan || !d_map || !d_slots || (!d_intervals && nintervals)) return fail(FFX_EINVAL, "search: null argument");
+    if (nintervals == 0 || totalSlots == 0) return FFX_OK;
+    if (!(slotSumBase & 1)) return fail(FFX_EINVAL, "search: slotSumBase must be odd");
+    if (lastSum > plan->sumLimitMax)
+        return fail(FFX_ERANGE, "search: interval sum %u above the plan's %u", lastSum, plan->sumLimitMax);
+    if (totalSlots > 0xFFFFFFFFull) return fail(FFX_ERANGE, "search: more than 2^32 slots");
+    if (maxPrimeMapValue < 15) return fail(FFX_EINVAL, "search: map too small");
+    SearchArgs a = {};
+    a.map = static_cast<const uint8_t *>(d_map);
+    a.M = maxPrimeMapValue;
+    a.nsums = (uint32_t)totalSlots;
+    a.slots = d_slots;
+    a.iv = d_intervals;
+    a.niv = nintervals;
+    a.bySum = 1;
+    a.sumBase = slotSumBase;
+    a.slotCap = slotCapacity;
+    return launch_search(plan, a, stream);
+}
+
+int ffx_search_plan_set(ffx_search_plan *plan, int param, uint32_t value)
+{
+    if (!plan) return fail(FFX_EINVAL, "search plan: null");
+    switch (param) {
+    case FFX_SEARCH_GRAB:
+        if (value < 1 || value > 64) return fail(FFX_ERANGE, "grab %u outside 1..64", value);
+        plan->grab = value;
+        return FFX_OK;
+    case FFX_SEARCH_EAGER_SPLITS:
+        plan->eager = value;
+        return FFX_OK;
+    case FFX_SEARCH_BLOCKS_PER_CU:
+        if (value < 1 || value > 16) return fail(FFX_ERANGE, "blocks per CU %u outside 1..16", value);
+        plan->blocksPerCU = value;
+        plan->grid = (int)(plan->cus * value);
+        return FFX_OK;
+    default:
+        return fail(FFX_EINVAL, "search plan: unknown parameter %d", param);
+    }
+}
+
+int ffx_search_plan_get(const ffx_search_plan *plan, int param, uint32_t *value)
+{
+    if (!plan || !value) return fail(FFX_EINVAL, "search plan: null");
+    switch (param) {
+    case FFX_SEARCH_GRAB: *value = plan->grab; return FFX_OK;
+    case FFX_SEARCH_EAGER_SPLITS: *value = plan->eager; return FFX_OK;
+    case FFX_SEARCH_BLOCKS_PER_CU: *value = plan->blocksPerCU; return FFX_OK;
+    default: return fail(FFX_EINVAL, "search plan: unknown parameter %d", param);
+    }
 }
 
 int ffx_compact_async(ffx_search_plan *plan, const uint32_t *d_slots, uint64_t nslots, uint32_t firstSum,

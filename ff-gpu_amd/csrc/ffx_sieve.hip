@@ -9,8 +9,9 @@
 // odd n <= M, untouched (1) for odd n > M, value 1 cleared.
 //
 // Kernel design (gfx950, wave64, 160 KiB LDS per CU):
-//  * One workgroup of 1024 threads owns one 64 KiB map segment at a time
-//    (2^19 odd values) in LDS and walks segments in a grid-stride loop.
+//  * One workgroup of 1024 threads owns one map segment at a time in LDS
+//    (64 KiB = 2^19 odd values by default, 32 KiB selectable) and walks
+//    segments in a grid-stride loop.
 //  * Bits are kept "struck" = 1 in LDS, in natural order (bit b of word w is
 //    odd index 32w+b); the write-out inverts them and reorders each byte to
 //    the reference's MSB-first layout (bit position b^7 within the word).
@@ -21,6 +22,7 @@
 //    multiple.  Larger primes are struck one prime per lane.  Strikes are
 //    LDS atomic ORs (ds_or_b32).
 //  * The segment leaves LDS through 16-byte coalesced stores.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -30,12 +32,9 @@
 
 namespace {
 
-constexpr uint32_t kSegBytes = 65536;
-constexpr uint32_t kSegWords = kSegBytes / 4;
-constexpr uint32_t kSegOdds = kSegBytes * 8;  // odd values per segment
+constexpr uint32_t kDefaultSegBytes = 65536;  // LDS tile: map bytes per segment
 constexpr int kThreads = 1024;
 constexpr int kWaves = kThreads / 64;
-constexpr uint32_t kCoopBelow = kSegOdds / 64;  // primes below strike wave-cooperatively
 
 constexpr int kGroups = 8;
 // Small-prime groups whose word patterns replace strikes.  Each period (the
@@ -65,7 +64,7 @@ struct SieveArgs {
     const uint32_t *primes;  // sieving primes > 61, ascending
     const double *invp;      // 1.0 / primes[i]
     uint32_t nprimes;
-    uint32_t ncoop;          // primes[0, ncoop) strike wave-cooperatively
+    uint32_t ncoop;          // primes[0, ncoop) strike wave-cooperatively (> 64 strikes per segment)
     const uint32_t *patterns;  // kPatWords words, groups back to back
     uint64_t segFirst, segCount;
 };
@@ -94,8 +93,11 @@ __device__ __forceinline__ uint32_t to_map_order(uint32_t v)
     return __builtin_bswap32(__builtin_bitreverse32(v));
 }
 
+template <uint32_t kSegBytes>
 __global__ __launch_bounds__(kThreads) void sieve_segments(SieveArgs a)
 {
+    constexpr uint32_t kSegWords = kSegBytes / 4;
+    constexpr uint32_t kSegOdds = kSegBytes * 8;  // odd values per segment
     __shared__ uint32_t seg[kSegWords];
     __shared__ uint32_t pat[kPatWords];
     const int tid = threadIdx.x;
@@ -228,12 +230,13 @@ uint64_t isqrt_u64(uint64_t n)
 
 struct ffx_sieve_plan {
     uint64_t maxPrimeMapValue;
-    uint32_t nprimes, ncoop;
+    uint32_t nprimes;
+    std::vector<uint32_t> hostPrimes;  // the struck primes (> 61)
+    uint32_t segBytes, blocksPerCU, cus;
     uint32_t *dPrimes;
     double *dInv;
     uint32_t *dPatterns;
     int device;
-    int grid;
 };
 
 using namespace ffx;
@@ -248,11 +251,7 @@ static int plan_build(ffx_sieve_plan **out, std::vector<uint32_t> &oddPrimes, ui
     for (uint32_t p : oddPrimes)
         if (p > kLargestPatternPrime && p <= sq) med.push_back(p);
     std::vector<double> inv(med.size());
-    uint32_t ncoop = 0;
-    for (size_t i = 0; i < med.size(); ++i) {
-        inv[i] = 1.0 / (double)med[i];
-        if (med[i] < kCoopBelow) ncoop = (uint32_t)i + 1;
-    }
+    for (size_t i = 0; i < med.size(); ++i) inv[i] = 1.0 / (double)med[i];
     // word patterns: bit b of word i of group g is set when odd index 32i+b
     // is congruent to (p-1)/2 mod some prime p of the group
     std::vector<uint32_t> pat(kPatWords, 0);
@@ -270,7 +269,9 @@ static int plan_build(ffx_sieve_plan **out, std::vector<uint32_t> &oddPrimes, ui
     if (!pl) return fail(FFX_ENOMEM, "sieve plan: host alloc");
     pl->maxPrimeMapValue = M;
     pl->nprimes = (uint32_t)med.size();
-    pl->ncoop = ncoop;
+    pl->hostPrimes = med;
+    pl->segBytes = kDefaultSegBytes;
+    pl->blocksPerCU = 1;
     hipError_t e = hipGetDevice(&pl->device);
     int cus = 0;
     if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, pl->device);
@@ -285,7 +286,7 @@ static int plan_build(ffx_sieve_plan **out, std::vector<uint32_t> &oddPrimes, ui
         ffx_sieve_plan_destroy(pl);
         return rc;
     }
-    pl->grid = cus > 0 ? cus : 256;  // one 1024-thread group per CU (LDS-bound)
+    pl->cus = cus > 0 ? (uint32_t)cus : 256;
     *out = pl;
     return FFX_OK;
 }
@@ -305,14 +306,20 @@ static int launch(const ffx_sieve_plan *pl, uint8_t *dst, uint64_t dstBase, uint
     a.primes = pl->dPrimes;
     a.invp = pl->dInv;
     a.nprimes = pl->nprimes;
-    a.ncoop = pl->ncoop;
+    const uint64_t segBytes = pl->segBytes, segOdds = segBytes * 8;
+    // primes with more than 64 strikes per segment strike wave-cooperatively
+    a.ncoop = (uint32_t)(std::lower_bound(pl->hostPrimes.begin(), pl->hostPrimes.end(), (uint32_t)(segOdds / 64)) -
+                         pl->hostPrimes.begin());
     a.patterns = pl->dPatterns;
-    a.segFirst = byteLo / kSegBytes;
-    a.segCount = (byteHi + kSegBytes - 1) / kSegBytes - a.segFirst;
-    if (a.segFirst * (uint64_t)kSegOdds >= (1ull << 52))
-        return fail(FFX_ERANGE, "sieve: map offset beyond 2^53 values");
-    const uint32_t grid = a.segCount < (uint64_t)pl->grid ? (uint32_t)a.segCount : (uint32_t)pl->grid;
-    hipLaunchKernelGGL(sieve_segments, dim3(grid), dim3(kThreads), 0, as_stream(stream), a);
+    a.segFirst = byteLo / segBytes;
+    a.segCount = (byteHi + segBytes - 1) / segBytes - a.segFirst;
+    if (a.segFirst * segOdds >= (1ull << 52)) return fail(FFX_ERANGE, "sieve: map offset beyond 2^53 values");
+    const uint64_t maxGrid = (uint64_t)pl->cus * pl->blocksPerCU;
+    const uint32_t grid = a.segCount < maxGrid ? (uint32_t)a.segCount : (uint32_t)maxGrid;
+    if (segBytes == 32768)
+        hipLaunchKernelGGL(sieve_segments<32768>, dim3(grid), dim3(kThreads), 0, as_stream(stream), a);
+    else
+        hipLaunchKernelGGL(sieve_segments<65536>, dim3(grid), dim3(kThreads), 0, as_stream(stream), a);
     FFX_HIP(hipGetLastError());
     return FFX_OK;
 }
@@ -361,6 +368,33 @@ int ffx_sieve_plan_create_from_primes(ffx_sieve_plan **out, const uint32_t *h_pr
             return fail(FFX_EINVAL, "sieve plan: prime list stops below isqrt(maxPrimeMapValue)");
     }
     return plan_build(out, primes, maxPrimeMapValue);
+}
+
+int ffx_sieve_plan_set(ffx_sieve_plan *plan, int param, uint32_t value)
+{
+    if (!plan) return fail(FFX_EINVAL, "sieve plan: null");
+    switch (param) {
+    case FFX_SIEVE_SEGMENT_BYTES:
+        if (value != 32768 && value != 65536) return fail(FFX_ERANGE, "segment bytes %u: 32768 or 65536", value);
+        plan->segBytes = value;
+        return FFX_OK;
+    case FFX_SIEVE_BLOCKS_PER_CU:
+        if (value < 1 || value > 8) return fail(FFX_ERANGE, "sieve blocks per CU %u outside 1..8", value);
+        plan->blocksPerCU = value;
+        return FFX_OK;
+    default:
+        return fail(FFX_EINVAL, "sieve plan: unknown parameter %d", param);
+    }
+}
+
+int ffx_sieve_plan_get(const ffx_sieve_plan *plan, int param, uint32_t *value)
+{
+    if (!plan || !value) return fail(FFX_EINVAL, "sieve plan: null");
+    switch (param) {
+    case FFX_SIEVE_SEGMENT_BYTES: *value = plan->segBytes; return FFX_OK;
+    case FFX_SIEVE_BLOCKS_PER_CU: *value = plan->blocksPerCU; return FFX_OK;
+    default: return fail(FFX_EINVAL, "sieve plan: unknown parameter %d", param);
+    }
 }
 
 int ffx_sieve_plan_destroy(ffx_sieve_plan *plan)

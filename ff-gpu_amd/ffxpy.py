@@ -43,8 +43,8 @@ class DeviceInfo(C.Structure):
                 ("memoryBusWidth", C.c_int32), ("runtimeIndex", C.c_int32), ("reserved", C.c_int32)]
 
 
-class BatchRange(C.Structure):
-    _fields_ = [("sumStart", C.c_uint32), ("sumLimit", C.c_uint32), ("slotOffset", C.c_uint64)]
+class Interval(C.Structure):
+    _fields_ = [("first", C.c_uint32), ("last", C.c_uint32), ("slotOffset", C.c_uint64)]
 
 
 _vp, _u32, _u64, _i32 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int
@@ -72,6 +72,7 @@ _SIGS = {
     "ffx_event_record": (_i32, [_vp, _vp]),
     "ffx_event_wait": (_i32, [_vp, _vp]),
     "ffx_event_sync": (_i32, [_vp]),
+    "ffx_event_query": (_i32, [_vp]),
     "ffx_event_elapsed_ms": (_i32, [_vp, _vp, C.POINTER(C.c_double)]),
     "ffx_device_sync": (_i32, []),
     "ffx_sieve_plan_create": (_i32, [C.POINTER(_vp), _u64]),
@@ -86,7 +87,20 @@ _SIGS = {
     "ffx_compact_async": (_i32, [_vp, _vp, _u64, _u32, _vp, _vp, _vp]),
     "ffx_search_async": (_i32, [_vp, _vp, _u64, _u32, _u32, _vp, _vp, _vp]),
     "ffx_search_records": (_i32, [_i32, _vp, _u64, _u64, _u64, _vp, _vp]),
+    "ffx_search_intervals_async": (_i32, [_vp, _vp, _u64, _vp, _u32, _u64, _u32, _vp, _vp]),
+    "ffx_search_intervals_by_sum_async": (_i32, [_vp, _vp, _u64, _vp, _u32, _u64, _u32, _u32, _vp, _u64, _vp]),
+    "ffx_search_plan_set": (_i32, [_vp, _i32, _u32]),
+    "ffx_search_plan_get": (_i32, [_vp, _i32, C.POINTER(_u32)]),
+    "ffx_sieve_plan_set": (_i32, [_vp, _i32, _u32]),
+    "ffx_sieve_plan_get": (_i32, [_vp, _i32, C.POINTER(_u32)]),
+    "ffx_graph_begin": (_i32, [_vp]),
+    "ffx_graph_end": (_i32, [_vp, C.POINTER(_vp)]),
+    "ffx_graph_launch": (_i32, [_vp, _vp]),
+    "ffx_graph_destroy": (_i32, [_vp]),
 }
+
+SEARCH_GRAB, SEARCH_EAGER_SPLITS, SEARCH_BLOCKS_PER_CU = 1, 2, 3
+SIEVE_SEGMENT_BYTES, SIEVE_BLOCKS_PER_CU = 1, 2
 
 _lib = None
 

@@ -222,6 +222,31 @@ int parseArgs(int argc, char **argv, Config *cfg, std::vector<std::string> *posi
             const char *v = need("--device");
             if (!v) return -1;
             cfg->device = std::atoi(v);
+        } else if (name == "--search-grab" || name == "--search-eager" || name == "--search-blocks-per-cu" ||
+                   name == "--sieve-segment" || name == "--sieve-blocks-per-cu") {
+            const char *v = need(name.c_str());
+            if (!v) return -1;
+            char *end = nullptr;
+            const unsigned long x = std::strtoul(v, &end, 10);
+            if (end == v || *end) {
+                std::fprintf(stderr, "[ff_sieve] error: invalid %s '%s'\n", name.c_str(), v);
+                return -1;
+            }
+            if (name == "--search-grab") cfg->searchGrab = (uint32_t)x;
+            else if (name == "--search-eager") cfg->searchEager = (uint32_t)x;
+            else if (name == "--search-blocks-per-cu") cfg->searchBlocksPerCU = (uint32_t)x;
+            else if (name == "--sieve-segment") cfg->sieveSegmentBytes = (uint32_t)x;
+            else cfg->sieveBlocksPerCU = (uint32_t)x;
+        } else if (name == "--tune") {
+            const char *v = need("--tune");
+            if (!v) return -1;
+            cfg->tuneFile = v;
+        } else if (name == "--batch" || name == "--serve" || name == "--autotune") {
+            cfg->mode = name.substr(2);
+        } else if (name == "--opt") {
+            const char *v = need("--opt");
+            if (!v) return -1;
+            cfg->modeArgs.push_back(v);
         } else if (name == "--dump-map") {
             const char *v = need("--dump-map");
             if (!v) return -1;
@@ -230,6 +255,41 @@ int parseArgs(int argc, char **argv, Config *cfg, std::vector<std::string> *posi
             std::fprintf(stderr, "[ff_sieve] error: unknown option '%s'\n", name.c_str());
             return -1;
         }
+    }
+    return 0;
+}
+
+int loadTuneFile(Config *cfg, const std::string &path)
+{
+    FILE *f = std::fopen(path.c_str(), "r");
+    if (!f) {
+        std::fprintf(stderr, "[ff_sieve] error: cannot read tuning file %s\n", path.c_str());
+        return -1;
+    }
+    char buf[4096];
+    const size_t n = std::fread(buf, 1, sizeof buf - 1, f);
+    std::fclose(f);
+    buf[n] = 0;
+    struct {
+        const char *key;
+        uint32_t *dst;
+    } keys[] = {{"search_grab", &cfg->searchGrab},
+                {"search_eager_splits", &cfg->searchEager},
+                {"search_blocks_per_cu", &cfg->searchBlocksPerCU},
+                {"sieve_segment_bytes", &cfg->sieveSegmentBytes},
+                {"sieve_blocks_per_cu", &cfg->sieveBlocksPerCU}};
+    for (auto &k : keys) {
+        const char *p = std::strstr(buf, k.key);
+        if (!p) continue;
+        p += std::strlen(k.key);
+        while (*p && (*p == '"' || *p == ':' || *p == '=' || *p == ' ')) ++p;
+        char *end = nullptr;
+        const unsigned long v = std::strtoul(p, &end, 10);
+        if (end == p) {
+            std::fprintf(stderr, "[ff_sieve] error: tuning file %s: bad value for %s\n", path.c_str(), k.key);
+            return -1;
+        }
+        if (*k.dst == 0) *k.dst = (uint32_t)v;  // explicit flags win
     }
     return 0;
 }
@@ -270,6 +330,7 @@ int validateConfig(Config *cfg)
         std::fprintf(stderr, "[ff_sieve] validation error: --primemem must be >= 16\n");
         return -1;
     }
+    if (!cfg->tuneFile.empty() && loadTuneFile(cfg, cfg->tuneFile) != 0) return -1;
     if (cfg->noHostTier) {
         cfg->hostTierCapBytes = 0;
         cfg->hostTierAuto = false;

@@ -38,7 +38,18 @@ struct Config {
     uint64_t primeMem = 85ull << 30;                // --primemem
     uint64_t mapCap = 0;                            // --map-cap (0 = ff_seg's M)
     int device = 0;                                 // --device
+    // kernel tuning (--tune FILE written by --autotune, or single flags)
+    uint32_t searchGrab = 0, searchEager = 0, searchBlocksPerCU = 0;  // 0 = library default
+    uint32_t sieveSegmentBytes = 0, sieveBlocksPerCU = 0;
+    std::string tuneFile;
+    // modes
+    std::string mode;                               // "" (plain), "batch", "serve", "autotune"
+    std::vector<std::string> modeArgs;              // key=value options of the mode
 };
+
+// Reads the key=value lines (or a flat JSON object of numbers) of a tuning
+// file into cfg's tuning fields.  Returns 0, or -1 with a message.
+int loadTuneFile(Config *cfg, const std::string &path);
 
 bool parseSize(const std::string &s, uint64_t *out);
 bool parseFraction(const std::string &s, double *out);

@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "batch.h"
 #include "budget.h"
 #include "config.h"
 #include "emission.h"
@@ -140,8 +141,46 @@ int main(int argc, char **argv)
     if (cmd == "served" && argc >= 4) {
         const double rate = argc >= 5 ? std::atof(argv[4]) : 1000.0;
         const std::vector<Request> r =
-            servedRequests(u64(argv[2]), (uint32_t)u64(argv[3]), 1u << 18, 1u << 23, 6.0, 18.0, rate);
+            servedRequests(u64(argv[2]), (uint32_t)u64(argv[3]), argc >= 6 ? (uint32_t)u64(argv[5]) : (1u << 18),
+                           argc >= 7 ? (uint32_t)u64(argv[6]) : (1u << 23), argc >= 8 ? std::atof(argv[7]) : 6.0,
+                           argc >= 9 ? std::atof(argv[8]) : 18.0, rate);
         for (const Request &x : r) std::printf("%u %u %.9f\n", x.sumStart, x.sumLimit, x.arrivalS);
+        return 0;
+    }
+    if (cmd == "spans") {
+        // stdin: "n m", n spans, m covered spans (odd lo <= hi).  Prints the
+        // union of the n spans as "first last slotOffset", a "--" line, the
+        // union minus the union of the covered spans, a "--" line, and the
+        // slot of every odd sum in [1, max + 6] (-1 when not covered).
+        size_t n = 0, m = 0;
+        if (std::scanf("%zu %zu", &n, &m) != 2) return 2;
+        std::vector<std::pair<uint32_t, uint32_t>> a(n), c(m);
+        uint32_t top = 0;
+        for (auto &x : a) {
+            if (std::scanf("%u %u", &x.first, &x.second) != 2) return 2;
+            if (x.second > top) top = x.second;
+        }
+        for (auto &x : c)
+            if (std::scanf("%u %u", &x.first, &x.second) != 2) return 2;
+        const IntervalSet U = unionOf(a), C = unionOf(c);
+        std::vector<std::pair<uint32_t, uint32_t>> u, cv;
+        for (const ffx_interval &x : U.iv) {
+            std::printf("%u %u %" PRIu64 "\n", x.first, x.last, x.slotOffset);
+            u.push_back({x.first, x.last});
+        }
+        for (const ffx_interval &x : C.iv) cv.push_back({x.first, x.last});
+        std::puts("--");
+        for (const auto &x : subtract(u, cv)) std::printf("%u %u\n", x.first, x.second);
+        std::puts("--");
+        for (uint32_t s = 1; s <= top + 6; s += 2) std::printf("%" PRId64 "\n", U.slotOf(s));
+        return 0;
+    }
+    if (cmd == "job" && argc >= 4) {
+        // makeJob: clamped range, chunk list and evaluated odd-sum span
+        const RangeJob j = makeJob((uint32_t)u64(argv[2]), (uint32_t)u64(argv[3]), argc >= 5 ? u64(argv[4]) : kSegPrimeMem);
+        std::printf("range %u %u\nfatal %d\nany %d\nspan %u %u\n", j.sumStart, j.sumLimit, (int)j.geom.fatal,
+                    (int)j.any, j.lo, j.hi);
+        for (const SumChunk &x : j.chunks) std::printf("chunk %u %u\n", x.lo, x.hi);
         return 0;
     }
     std::fprintf(stderr, "ff_hostcheck: bad command\n");

@@ -19,6 +19,7 @@
 #include "emission.h"
 #include "ffx.h"
 #include "geometry.h"
+#include "modes.h"
 #include "partition.h"
 #include "scheduler.h"
 
@@ -159,6 +160,10 @@ int runLeg(const Config &cfg, const std::vector<std::string> &pos)
     if (!stream) return err("stream", FFX_EIO);
     ffx_sieve_plan *sp = nullptr;
     if ((rc = ffx_sieve_plan_create(&sp, mapM))) return err("sieve plan", rc);
+    if (cfg.sieveSegmentBytes && (rc = ffx_sieve_plan_set(sp, FFX_SIEVE_SEGMENT_BYTES, cfg.sieveSegmentBytes)))
+        return err("sieve tuning", rc);
+    if (cfg.sieveBlocksPerCU && (rc = ffx_sieve_plan_set(sp, FFX_SIEVE_BLOCKS_PER_CU, cfg.sieveBlocksPerCU)))
+        return err("sieve tuning", rc);
     void *dMap = ffx_alloc(((mapBytes + 255) & ~255ull));
     if (!dMap) return err("map allocation", FFX_ENOMEM);
     std::vector<uint8_t> hostMap;
@@ -195,6 +200,10 @@ int runLeg(const Config &cfg, const std::vector<std::string> &pos)
         if (hi > FFX_MAX_SUM) return err("search range", FFX_ERANGE);
         ffx_search_plan *plan = nullptr;
         if ((rc = ffx_search_plan_create(&plan, hi))) return err("search plan", rc);
+        if ((cfg.searchGrab && (rc = ffx_search_plan_set(plan, FFX_SEARCH_GRAB, cfg.searchGrab))) ||
+            (cfg.searchEager && (rc = ffx_search_plan_set(plan, FFX_SEARCH_EAGER_SPLITS, cfg.searchEager))) ||
+            (cfg.searchBlocksPerCU && (rc = ffx_search_plan_set(plan, FFX_SEARCH_BLOCKS_PER_CU, cfg.searchBlocksPerCU))))
+            return err("search tuning", rc);
         const uint64_t nslots = (uint64_t)(hi - lo) / 2 + 1;
         ffx_solution *dOut = (ffx_solution *)ffx_alloc(nslots * sizeof(ffx_solution));
         uint32_t *dCount = (uint32_t *)ffx_alloc(4);
@@ -231,6 +240,42 @@ int runLeg(const Config &cfg, const std::vector<std::string> &pos)
     return 0;
 }
 
+// key=value options of --batch / --serve (--opt key=value, repeatable)
+bool modeOptions(const Config &cfg, BatchOptions *b, ServeOptions *sv)
+{
+    for (const std::string &kv : cfg.modeArgs) {
+        const size_t eq = kv.find('=');
+        if (eq == std::string::npos) {
+            std::fprintf(stderr, "[ff_sieve] error: --opt %s is not key=value\n", kv.c_str());
+            return false;
+        }
+        const std::string k = kv.substr(0, eq), v = kv.substr(eq + 1);
+        const double d = std::atof(v.c_str());
+        const unsigned long long u = std::strtoull(v.c_str(), nullptr, 10);
+        if (k == "ranges") b->rangesFile = v;
+        else if (k == "seed") b->seed = sv->seed = u;
+        else if (k == "count") b->count = sv->count = (uint32_t)u;
+        else if (k == "start_max") b->startMax = (uint32_t)u;
+        else if (k == "wlo") b->wLo = sv->wLo = d;
+        else if (k == "whi") b->wHi = sv->wHi = d;
+        else if (k == "out_dir") b->outDir = sv->outDir = v;
+        else if (k == "reps") b->reps = (uint32_t)u;
+        else if (k == "graph") b->useGraph = u != 0;
+        else if (k == "lim_lo") sv->limLo = (uint32_t)u;
+        else if (k == "lim_hi") sv->limHi = (uint32_t)u;
+        else if (k == "rate") sv->rate = d;
+        else if (k == "max_batch") sv->maxBatchRequests = (uint32_t)u;
+        else if (k == "streams") sv->streams = (uint32_t)u;
+        else if (k == "plain_sample") sv->plainSample = (uint32_t)u;
+        else if (k == "json") sv->jsonOut = v;
+        else {
+            std::fprintf(stderr, "[ff_sieve] error: unknown --opt key '%s'\n", k.c_str());
+            return false;
+        }
+    }
+    return true;
+}
+
 }  // namespace
 
 int main(int argc, char **argv)
@@ -242,7 +287,12 @@ int main(int argc, char **argv)
         std::vector<std::string> pos;
         if (loadEnv(&cfg) || parseArgs(argc, argv, &cfg, &pos) || validateConfig(&cfg)) ret = 1;
         else if (cfg.listDevices) ret = listDevices();
-        else ret = runLeg(cfg, pos);  // no positionals: ff_seg's default leg 5..2627
+        else if (cfg.mode == "batch" || cfg.mode == "serve") {
+            BatchOptions b;
+            ServeOptions sv;
+            if (!modeOptions(cfg, &b, &sv)) ret = 1;
+            else ret = cfg.mode == "batch" ? runBatch(cfg, b) : runServe(cfg, sv);
+        } else ret = runLeg(cfg, pos);  // no positionals: ff_seg's default leg 5..2627
     }
     phase("total", msSince(t0));
     return ret;

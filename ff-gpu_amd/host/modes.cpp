@@ -1,0 +1,647 @@
+// Batched and served modes of ff_sieve.
+//
+// --batch: N sub-ranges, each answered exactly as `ff_sieve <a> <b>` would
+// answer it, from ONE sieve and ONE persistent search launch over the union
+// of their sums (captured in a hipGraph).
+//
+// --serve: a seeded Poisson stream of (sumStart, sumLimit) requests.  The
+// bitmap stays resident and grows on demand; requests that have arrived are
+// admitted while the device-memory budget allows, batched into persistent
+// launches over the sums no earlier batch has computed, and up to `streams`
+// batches run on the device at once.  Per-request latency is arrival to
+// output; the same requests can be replayed one at a time as plain launches
+// for comparison.
+#include "modes.h"
+
+#include <sys/stat.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <deque>
+#include <fstream>
+#include <functional>
+#include <sstream>
+#include <condition_variable>
+#include <mutex>
+#include <thread>
+
+#include "batch.h"
+#include "budget.h"
+#include "ffx.h"
+#include "workload.h"
+
+namespace ffg {
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double msSince(Clock::time_point t) { return std::chrono::duration<double, std::milli>(Clock::now() - t).count(); }
+
+int fail(const char *what, int rc)
+{
+    std::fprintf(stderr, "[ff_sieve] error: %s failed (%d): %s\n", what, rc, ffx_last_error());
+    return 1;
+}
+
+// Device map value for a set of jobs: the largest ff_seg M among them, or
+// the --map-cap bound (Miller-Rabin answers above the map either way).
+uint64_t mapValueFor(uint64_t maxM, uint64_t cap)
+{
+    uint64_t M = maxM;
+    if (cap && cap < M) M = cap;
+    M = ((M + 1) & ~15ull) - 1;
+    return M < 15 ? 15 : M;
+}
+
+int applyTuning(const Config &cfg, ffx_sieve_plan *sp, ffx_search_plan *qp)
+{
+    int rc = 0;
+    if (sp && cfg.sieveSegmentBytes && (rc = ffx_sieve_plan_set(sp, FFX_SIEVE_SEGMENT_BYTES, cfg.sieveSegmentBytes)))
+        return rc;
+    if (sp && cfg.sieveBlocksPerCU && (rc = ffx_sieve_plan_set(sp, FFX_SIEVE_BLOCKS_PER_CU, cfg.sieveBlocksPerCU)))
+        return rc;
+    if (qp && cfg.searchGrab && (rc = ffx_search_plan_set(qp, FFX_SEARCH_GRAB, cfg.searchGrab))) return rc;
+    if (qp && cfg.searchEager && (rc = ffx_search_plan_set(qp, FFX_SEARCH_EAGER_SPLITS, cfg.searchEager))) return rc;
+    if (qp && cfg.searchBlocksPerCU && (rc = ffx_search_plan_set(qp, FFX_SEARCH_BLOCKS_PER_CU, cfg.searchBlocksPerCU)))
+        return rc;
+    return 0;
+}
+
+bool writeFile(const std::string &path, const std::string &data)
+{
+    FILE *f = std::fopen(path.c_str(), "wb");
+    if (!f) return false;
+    const bool ok = std::fwrite(data.data(), 1, data.size(), f) == data.size();
+    return std::fclose(f) == 0 && ok;
+}
+
+std::string fatalText(const RangeJob &j)
+{
+    return j.geom.headers[0] +
+           "\n\nThe input parameters would indicate that more memory is required. Please adjust input parameters or "
+           "Prime::maxPrimeAlloc.\n\n";
+}
+
+double percentile(std::vector<double> v, double p)
+{
+    if (v.empty()) return 0;
+    std::sort(v.begin(), v.end());
+    const double idx = p * (double)(v.size() - 1);
+    const size_t i = (size_t)idx;
+    const double f = idx - (double)i;
+    return i + 1 < v.size() ? v[i] * (1 - f) + v[i + 1] * f : v[i];
+}
+
+struct DeviceSetup {
+    ffx_device_info info{};
+    DeviceBudget budget;
+};
+
+int setupDevice(const Config &cfg, DeviceSetup *d)
+{
+    int rc = ffx_init(cfg.device);
+    if (rc) return fail("device init", rc);
+    if ((rc = ffx_device_info_get(cfg.device, &d->info))) return fail("device info", rc);
+    d->budget = computeBudget(d->info.freeBytes, cfg, cfg.globalFraction);
+    return 0;
+}
+
+}  // namespace
+
+// --------------------------------------------------------------------------
+// batched
+// --------------------------------------------------------------------------
+
+int runBatch(const Config &cfg, const BatchOptions &opt)
+{
+    std::vector<SumRange> ranges;
+    if (!opt.rangesFile.empty()) {
+        std::ifstream in(opt.rangesFile);
+        if (!in) {
+            std::fprintf(stderr, "[ff_sieve] error: cannot read %s\n", opt.rangesFile.c_str());
+            return 1;
+        }
+        unsigned long long a, b;
+        while (in >> a >> b) ranges.push_back({(uint32_t)a, (uint32_t)b});
+    } else {
+        ranges = batchRanges(opt.seed, opt.count, opt.startMax, opt.wLo, opt.wHi);
+    }
+    std::vector<RangeJob> jobs;
+    jobs.reserve(ranges.size());
+    std::vector<std::pair<uint32_t, uint32_t>> spans;
+    uint64_t maxM = 15, requested = 0;
+    for (const SumRange &r : ranges) {
+        jobs.push_back(makeJob(r.sumStart, r.sumLimit, cfg.primeMem));
+        const RangeJob &j = jobs.back();
+        if (j.geom.fatal) continue;
+        if (j.geom.maxPrimeMapValue > maxM) maxM = j.geom.maxPrimeMapValue;
+        for (const SumChunk &c : j.chunks)
+            if ((c.lo | 1u) <= c.hi) requested += (c.hi - (c.lo | 1u)) / 2 + 1;
+        if (j.any) spans.push_back({j.lo, j.hi});
+    }
+    const IntervalSet U = unionOf(spans);
+    if (U.lastSum() > FFX_MAX_SUM) {
+        std::fprintf(stderr, "[ff_sieve] error: sums above %u\n", FFX_MAX_SUM);
+        return 1;
+    }
+    DeviceSetup dev;
+    if (setupDevice(cfg, &dev)) return 1;
+    const uint64_t M = mapValueFor(maxM, cfg.mapCap);
+    const uint64_t mapBytes = (M >> 4) + 1;
+    const uint64_t need = mapBytes + U.totalSlots * 4 + U.iv.size() * sizeof(ffx_interval);
+    std::fprintf(stderr,
+                 "[ff_sieve] batch: %zu ranges, %llu requested odd sums, %llu distinct in %zu intervals; map M=%llu "
+                 "(%s), backing %s\n",
+                 jobs.size(), (unsigned long long)requested, (unsigned long long)U.totalSlots, U.iv.size(),
+                 (unsigned long long)M, bytesToHuman(mapBytes).c_str(), bytesToHuman(dev.budget.backing).c_str());
+    if (need > dev.budget.backing) {
+        std::fprintf(stderr, "[ff_sieve] ERROR: batch needs %s > backing %s; bound the map with --map-cap\n",
+                     bytesToHuman(need).c_str(), bytesToHuman(dev.budget.backing).c_str());
+        return 1;
+    }
+
+    void *stream = ffx_stream_create();
+    void *ev[4] = {ffx_event_create(), ffx_event_create(), ffx_event_create(), ffx_event_create()};
+    if (!stream || !ev[0] || !ev[1] || !ev[2] || !ev[3]) return fail("stream/events", FFX_EIO);
+    int rc;
+    ffx_sieve_plan *sp = nullptr;
+    ffx_search_plan *qp = nullptr;
+    if ((rc = ffx_sieve_plan_create(&sp, M))) return fail("sieve plan", rc);
+    if ((rc = ffx_search_plan_create(&qp, U.lastSum() > 5 ? U.lastSum() : 5))) return fail("search plan", rc);
+    if ((rc = applyTuning(cfg, sp, qp))) return fail("tuning", rc);
+    void *dMap = ffx_alloc((mapBytes + 255) & ~255ull);
+    uint32_t *dSlots = (uint32_t *)ffx_alloc(U.totalSlots * 4 + 4);
+    ffx_interval *dIv = (ffx_interval *)ffx_alloc(U.iv.size() * sizeof(ffx_interval) + 16);
+    if (!dMap || !dSlots || !dIv) return fail("allocation", FFX_ENOMEM);
+    if (!U.iv.empty() && (rc = ffx_copy_h2d(dIv, U.iv.data(), U.iv.size() * sizeof(ffx_interval))))
+        return fail("interval upload", rc);
+
+    // sieve once
+    const auto t0 = Clock::now();
+    ffx_event_record(ev[0], stream);
+    if ((rc = ffx_sieve_map_async(sp, dMap, mapBytes, stream))) return fail("sieve", rc);
+    ffx_event_record(ev[1], stream);
+    if ((rc = ffx_event_sync(ev[1]))) return fail("sieve sync", rc);
+    double sieveMs = 0;
+    ffx_event_elapsed_ms(ev[0], ev[1], &sieveMs);
+    const double sieveWall = msSince(t0);
+
+    // one persistent launch over every interval, captured once, replayed
+    ffx_graph *graph = nullptr;
+    if (opt.useGraph && !U.iv.empty()) {
+        if ((rc = ffx_graph_begin(stream))) return fail("graph capture", rc);
+        rc = ffx_search_intervals_async(qp, dMap, M, dIv, (uint32_t)U.iv.size(), U.totalSlots, U.lastSum(), dSlots,
+                                        stream);
+        ffx_graph *g = nullptr;
+        const int rc2 = ffx_graph_end(stream, &g);
+        if (rc) return fail("batched search (capture)", rc);
+        if (rc2) return fail("graph instantiate", rc2);
+        graph = g;
+    }
+    std::vector<double> runs;
+    const auto tS = Clock::now();
+    for (uint32_t r = 0; r < (opt.reps ? opt.reps : 1); ++r) {
+        ffx_event_record(ev[2], stream);
+        if (graph) rc = ffx_graph_launch(graph, stream);
+        else if (!U.iv.empty())
+            rc = ffx_search_intervals_async(qp, dMap, M, dIv, (uint32_t)U.iv.size(), U.totalSlots, U.lastSum(),
+                                            dSlots, stream);
+        if (rc) return fail("batched search", rc);
+        ffx_event_record(ev[3], stream);
+        if ((rc = ffx_event_sync(ev[3]))) return fail("search sync", rc);
+        double ms = 0;
+        ffx_event_elapsed_ms(ev[2], ev[3], &ms);
+        runs.push_back(ms);
+    }
+    const double searchWall = msSince(tS) / (double)(opt.reps ? opt.reps : 1);
+    std::vector<uint32_t> slots(U.totalSlots + 1);
+    if (U.totalSlots && (rc = ffx_copy_d2h(slots.data(), dSlots, U.totalSlots * 4))) return fail("slot copy", rc);
+    const double searchMs = *std::min_element(runs.begin(), runs.end());
+
+    // per-range stdout, each exactly a plain run's
+    const auto tE = Clock::now();
+    uint64_t lines = 0;
+    if (!opt.outDir.empty()) mkdir(opt.outDir.c_str(), 0755);
+    std::string out;
+    for (size_t i = 0; i < jobs.size(); ++i) {
+        out.clear();
+        if (jobs[i].geom.fatal) out = fatalText(jobs[i]);
+        else
+            lines += emitJob(out, jobs[i], [&](uint32_t s) {
+                const int64_t k = U.slotOf(s);
+                return k < 0 ? 0u : slots[(size_t)k];
+            }, (uint64_t)(sieveWall * 1000), (uint64_t)(searchWall * 1000));
+        if (!opt.outDir.empty()) {
+            char name[64];
+            std::snprintf(name, sizeof name, "/range_%05zu.txt", i);
+            if (!writeFile(opt.outDir + name, out)) {
+                std::fprintf(stderr, "[ff_sieve] error: cannot write %s%s\n", opt.outDir.c_str(), name);
+                return 1;
+            }
+        }
+    }
+    const double emitMs = msSince(tE);
+    std::printf("{\"mode\": \"batch\", \"ranges\": %zu, \"requested_odd_sums\": %llu, \"distinct_odd_sums\": %llu, "
+                "\"intervals\": %zu, \"solution_lines\": %llu, \"map_value\": %llu, \"map_bytes\": %llu, "
+                "\"sieve_ms\": %.3f, \"search_ms\": %.3f, \"emit_ms\": %.3f, \"graph\": %s, "
+                "\"requested_sums_per_s\": %.1f, \"distinct_sums_per_s\": %.1f, \"sieve_gbs\": %.1f}\n",
+                jobs.size(), (unsigned long long)requested, (unsigned long long)U.totalSlots, U.iv.size(),
+                (unsigned long long)lines, (unsigned long long)M, (unsigned long long)mapBytes, sieveMs, searchMs,
+                emitMs, graph ? "true" : "false", (double)requested / ((sieveMs + searchMs) * 1e-3),
+                (double)U.totalSlots / (searchMs * 1e-3), (double)mapBytes / (sieveMs * 1e6));
+    ffx_graph_destroy(graph);
+    ffx_free(dIv);
+    ffx_free(dSlots);
+    ffx_free(dMap);
+    ffx_search_plan_destroy(qp);
+    ffx_sieve_plan_destroy(sp);
+    for (void *e : ev) ffx_event_destroy(e);
+    ffx_stream_destroy(stream);
+    return 0;
+}
+
+// --------------------------------------------------------------------------
+// served
+// --------------------------------------------------------------------------
+
+namespace {
+
+using Spans = std::vector<std::pair<uint32_t, uint32_t>>;
+
+// Merges b into the sorted disjoint odd-sum span list a.
+void mergeInto(Spans &a, const Spans &b)
+{
+    if (b.empty()) return;
+    Spans all(a);
+    all.insert(all.end(), b.begin(), b.end());
+    std::sort(all.begin(), all.end());
+    a.clear();
+    for (const auto &x : all) {
+        if (!a.empty() && (uint64_t)x.first <= (uint64_t)a.back().second + 2) a.back().second = std::max(a.back().second, x.second);
+        else a.push_back(x);
+    }
+}
+
+bool covers(const Spans &a, uint32_t lo, uint32_t hi)
+{
+    auto it = std::upper_bound(a.begin(), a.end(), std::make_pair(lo, 0xFFFFFFFFu));
+    if (it == a.begin()) return false;
+    --it;
+    return it->first <= lo && it->second >= hi;
+}
+
+// Output formatting pool: completed requests are formatted off the
+// scheduling thread; completion time is when the text is ready.
+struct EmitPool {
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<size_t> q;
+    bool done = false;
+    std::vector<std::thread> th;
+    std::function<void(size_t)> work;
+    void start(int n, std::function<void(size_t)> w)
+    {
+        work = std::move(w);
+        for (int i = 0; i < n; ++i)
+            th.emplace_back([this] {
+                for (;;) {
+                    size_t id;
+                    {
+                        std::unique_lock<std::mutex> lk(mu);
+                        cv.wait(lk, [this] { return done || !q.empty(); });
+                        if (q.empty()) return;
+                        id = q.front();
+                        q.pop_front();
+                    }
+                    work(id);
+                }
+            });
+    }
+    void push(size_t id)
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            q.push_back(id);
+        }
+        cv.notify_one();
+    }
+    ~EmitPool() { finish(); }
+    void finish()
+    {
+        if (th.empty()) return;
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            done = true;
+        }
+        cv.notify_all();
+        for (auto &t : th) t.join();
+        th.clear();
+    }
+};
+
+struct InFlight {
+    int stream;
+    Spans spans;
+};
+
+}  // namespace
+
+int runServe(const Config &cfg, const ServeOptions &opt)
+{
+    const std::vector<Request> reqs =
+        servedRequests(opt.seed, opt.count, opt.limLo, opt.limHi, opt.wLo, opt.wHi, opt.rate);
+    const size_t n = reqs.size();
+    std::vector<RangeJob> jobs(n);
+    uint64_t maxM = 15, requested = 0;
+    uint32_t maxSum = 5;
+    for (size_t i = 0; i < n; ++i) {
+        jobs[i] = makeJob(reqs[i].sumStart, reqs[i].sumLimit, cfg.primeMem);
+        const RangeJob &j = jobs[i];
+        if (j.geom.fatal || !j.any) continue;
+        maxM = std::max(maxM, j.geom.maxPrimeMapValue);
+        maxSum = std::max(maxSum, j.hi);
+        requested += (j.hi - j.lo) / 2 + 1;
+    }
+    if (maxSum > FFX_MAX_SUM) {
+        std::fprintf(stderr, "[ff_sieve] error: sums above %u\n", FFX_MAX_SUM);
+        return 1;
+    }
+    DeviceSetup dev;
+    if (setupDevice(cfg, &dev)) return 1;
+    const int S = (int)std::max(1u, std::min(opt.streams, 4u));
+    const uint32_t base = 5;
+    const uint64_t nslots = (maxSum - base) / 2 + 1;
+    const uint32_t ivCap = 1u << 16;  // intervals per batch
+    const uint64_t fixedBytes = nslots * 4 + (uint64_t)S * ivCap * sizeof(ffx_interval);
+    // admission against the device budget (source/budget.cpp:15-42): the
+    // resident map gets what the backing holds after the fixed buffers
+    uint64_t M = mapValueFor(maxM, cfg.mapCap);
+    if (dev.budget.backing < fixedBytes + (64ull << 20)) {
+        std::fprintf(stderr, "[ff_sieve] ERROR: budget %s cannot hold the serving buffers\n",
+                     bytesToHuman(dev.budget.backing).c_str());
+        return 1;
+    }
+    const uint64_t mapRoom = dev.budget.backing - fixedBytes;
+    if ((M >> 4) + 1 > mapRoom) M = mapValueFor(((mapRoom - 1) << 4) - 1, 0);
+    const uint64_t mapBytesMax = (M >> 4) + 1;
+    std::fprintf(stderr,
+                 "[ff_sieve] serve: %zu requests, %llu requested odd sums up to %u; resident map up to M=%llu (%s, "
+                 "grown on demand), budget backing %s, %d streams\n",
+                 n, (unsigned long long)requested, maxSum, (unsigned long long)M, bytesToHuman(mapBytesMax).c_str(),
+                 bytesToHuman(dev.budget.backing).c_str(), S);
+
+    int rc;
+    ffx_sieve_plan *sp = nullptr;
+    if ((rc = ffx_sieve_plan_create(&sp, M))) return fail("sieve plan", rc);
+    std::vector<ffx_search_plan *> qp(S, nullptr);
+    std::vector<void *> st(S), ev(S);
+    std::vector<ffx_interval *> dIv(S), hIv(S);
+    for (int k = 0; k < S; ++k) {
+        if ((rc = ffx_search_plan_create(&qp[k], maxSum))) return fail("search plan", rc);
+        if ((rc = applyTuning(cfg, k == 0 ? sp : nullptr, qp[k]))) return fail("tuning", rc);
+        st[k] = ffx_stream_create();
+        ev[k] = ffx_event_create();
+        dIv[k] = (ffx_interval *)ffx_alloc((uint64_t)ivCap * sizeof(ffx_interval));
+        hIv[k] = (ffx_interval *)ffx_alloc_pinned((uint64_t)ivCap * sizeof(ffx_interval));
+        if (!st[k] || !ev[k] || !dIv[k] || !hIv[k]) return fail("stream resources", FFX_ENOMEM);
+    }
+    void *dMap = ffx_alloc((mapBytesMax + 255) & ~255ull);
+    uint32_t *dSlots = (uint32_t *)ffx_alloc(nslots * 4);
+    uint32_t *mirror = (uint32_t *)ffx_alloc_pinned(nslots * 4);
+    if (!dMap || !dSlots || !mirror) return fail("allocation", FFX_ENOMEM);
+
+    std::vector<double> doneAt(n, -1.0);
+    std::vector<char> admitted(n, 0);
+    std::vector<size_t> waiting;  // admitted, output not yet started
+    Spans computed, inflightSpans;
+    std::deque<InFlight> inflight;
+    std::vector<bool> busy(S, false);
+    uint64_t mapBytesDone = 0, batches = 0, searchedSlots = 0, lines = 0, growths = 0;
+    std::mutex linesMu;
+    double primeMs = 0;
+    const bool writeOut = !opt.outDir.empty();
+    if (writeOut) mkdir(opt.outDir.c_str(), 0755);
+
+    const auto t0 = Clock::now();
+    auto now = [&] { return std::chrono::duration<double>(Clock::now() - t0).count(); };
+    EmitPool pool;
+    pool.start(8, [&](size_t id) {
+        std::string out;
+        uint64_t nl = 0;
+        if (jobs[id].geom.fatal) out = fatalText(jobs[id]);
+        else
+            nl = emitJob(out, jobs[id], [&](uint32_t s) { return mirror[(s - base) >> 1]; },
+                         (uint64_t)(primeMs * 1000), 0);
+        if (writeOut) {
+            char name[64];
+            std::snprintf(name, sizeof name, "/request_%05zu.txt", id);
+            writeFile(opt.outDir + name, out);
+        }
+        doneAt[id] = now();
+        std::lock_guard<std::mutex> lk(linesMu);
+        lines += nl;
+    });
+
+    size_t next = 0, completed = 0;
+    std::deque<size_t> pending;
+    while (completed < n) {
+        bool progress = false;
+        const double t = now();
+        while (next < n && reqs[next].arrivalS <= t) pending.push_back(next++);
+        // launch batches while a stream is free
+        while (!pending.empty()) {
+            int k = -1;
+            for (int i = 0; i < S; ++i)
+                if (!busy[i]) {
+                    k = i;
+                    break;
+                }
+            if (k < 0) break;
+            Spans want;
+            uint64_t needM = 15;
+            size_t taken = 0;
+            while (!pending.empty() && taken < opt.maxBatchRequests) {
+                const size_t id = pending.front();
+                pending.pop_front();
+                ++taken;
+                admitted[id] = 1;
+                waiting.push_back(id);
+                const RangeJob &j = jobs[id];
+                if (j.geom.fatal || !j.any) continue;
+                want.push_back({j.lo, j.hi});
+                needM = std::max(needM, std::min(j.geom.maxPrimeMapValue, M));
+            }
+            Spans wantU;
+            mergeInto(wantU, want);
+            Spans missing = subtract(subtract(wantU, computed), inflightSpans);
+            progress = true;
+            if (missing.empty()) continue;
+            // the batch's intervals, in slot order
+            std::vector<ffx_interval> iv;
+            uint64_t total = 0;
+            for (const auto &m : missing) {
+                ffx_interval x;
+                x.first = m.first;
+                x.last = m.second;
+                x.slotOffset = total;
+                total += (m.second - m.first) / 2 + 1;
+                iv.push_back(x);
+            }
+            if (iv.size() > ivCap) {
+                std::fprintf(stderr, "[ff_sieve] error: %zu intervals in one batch (max %u)\n", iv.size(), ivCap);
+                return 1;
+            }
+            // grow the resident map to this batch's largest geometry
+            // (whole GiB steps; the host waits for the growth, so batches
+            // later launched on other streams see the new bytes)
+            uint64_t needBytes = (needM >> 4) + 1;
+            if (needBytes > mapBytesDone) {
+                needBytes = std::min<uint64_t>(mapBytesMax, (needBytes + (1ull << 30) - 1) & ~((1ull << 30) - 1));
+                const auto tg = Clock::now();
+                if ((rc = ffx_sieve_map_range_async(sp, dMap, mapBytesDone, needBytes, st[k])) ||
+                    (rc = ffx_stream_sync(st[k])))
+                    return fail("map growth", rc);
+                mapBytesDone = needBytes;
+                primeMs += msSince(tg);
+                ++growths;
+            }
+            const uint64_t Muse = std::min(M, (mapBytesDone << 4) - 1);
+            std::memcpy(hIv[k], iv.data(), iv.size() * sizeof(ffx_interval));
+            if ((rc = ffx_copy_h2d_async(dIv[k], hIv[k], iv.size() * sizeof(ffx_interval), st[k])))
+                return fail("interval upload", rc);
+            if ((rc = ffx_search_intervals_by_sum_async(qp[k], dMap, Muse, dIv[k], (uint32_t)iv.size(), total,
+                                                        iv.back().last, base, dSlots, nslots, st[k])))
+                return fail("served search", rc);
+            for (const ffx_interval &x : iv) {
+                const uint64_t off = (x.first - base) >> 1, cnt = (x.last - x.first) / 2 + 1;
+                if ((rc = ffx_copy_d2h_async(mirror + off, dSlots + off, cnt * 4, st[k]))) return fail("slot copy", rc);
+            }
+            if ((rc = ffx_event_record(ev[k], st[k]))) return fail("event", rc);
+            busy[k] = true;
+            inflight.push_back({k, missing});
+            mergeInto(inflightSpans, missing);
+            ++batches;
+            searchedSlots += total;
+        }
+        // completed batches
+        bool landed = false;
+        for (auto it = inflight.begin(); it != inflight.end();) {
+            const int q = ffx_event_query(ev[it->stream]);
+            if (q < 0) return fail("event query", q);
+            if (q == 0) {
+                mergeInto(computed, it->spans);
+                busy[it->stream] = false;
+                it = inflight.erase(it);
+                landed = true;
+            } else {
+                ++it;
+            }
+        }
+        if (landed) {
+            inflightSpans.clear();
+            for (const InFlight &f : inflight) mergeInto(inflightSpans, f.spans);
+            progress = true;
+        }
+        // requests whose sums are all computed go to the formatting pool
+        for (size_t i = 0; i < waiting.size();) {
+            const size_t id = waiting[i];
+            const RangeJob &j = jobs[id];
+            if (j.geom.fatal || !j.any || covers(computed, j.lo, j.hi)) {
+                pool.push(id);
+                ++completed;
+                waiting[i] = waiting.back();
+                waiting.pop_back();
+                progress = true;
+            } else {
+                ++i;
+            }
+        }
+        if (!progress) std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
+    pool.finish();
+    const double makespan = now();
+    std::vector<double> lat(n);
+    for (size_t i = 0; i < n; ++i) lat[i] = (doneAt[i] - reqs[i].arrivalS) * 1000.0;
+    const double p50 = percentile(lat, 0.50), p99 = percentile(lat, 0.99);
+
+    // sequential plain launches of a sample, same arrival times, FIFO, one
+    // request at a time, no sharing between requests, map resident
+    double pp50 = 0, pp99 = 0, plainRate = 0;
+    uint64_t plainSums = 0;
+    if (opt.plainSample) {
+        const size_t m = std::min<size_t>(opt.plainSample, n);
+        std::vector<double> plat;
+        uint32_t *dS = (uint32_t *)ffx_alloc(nslots * 4);
+        if (!dS) return fail("plain slots", FFX_ENOMEM);
+        if (mapBytesDone < mapBytesMax) {
+            if ((rc = ffx_sieve_map_range_async(sp, dMap, mapBytesDone, mapBytesMax, st[0]))) return fail("map", rc);
+            ffx_stream_sync(st[0]);
+            mapBytesDone = mapBytesMax;
+        }
+        const auto p0 = Clock::now();
+        double clock = 0;  // seconds since the sample's first arrival
+        const double a0 = reqs[0].arrivalS;
+        for (size_t i = 0; i < m; ++i) {
+            const RangeJob &j = jobs[i];
+            const double arr = reqs[i].arrivalS - a0;
+            const double nowS = std::chrono::duration<double>(Clock::now() - p0).count();
+            if (nowS < arr) std::this_thread::sleep_for(std::chrono::duration<double>(arr - nowS));
+            if (!j.geom.fatal && j.any) {
+                ffx_interval x;
+                x.first = j.lo;
+                x.last = j.hi;
+                x.slotOffset = 0;
+                const uint64_t cnt = (j.hi - j.lo) / 2 + 1;
+                std::memcpy(hIv[0], &x, sizeof x);
+                ffx_copy_h2d_async(dIv[0], hIv[0], sizeof x, st[0]);
+                if ((rc = ffx_search_intervals_by_sum_async(qp[0], dMap, M, dIv[0], 1, cnt, j.hi, base, dS, nslots,
+                                                            st[0])))
+                    return fail("plain search", rc);
+                const uint64_t off = (j.lo - base) >> 1;
+                ffx_copy_d2h_async(mirror + off, dS + off, cnt * 4, st[0]);
+                ffx_stream_sync(st[0]);
+                plainSums += cnt;
+            }
+            std::string out;
+            if (!j.geom.fatal) emitJob(out, j, [&](uint32_t s) { return mirror[(s - base) >> 1]; }, 0, 0);
+            clock = std::chrono::duration<double>(Clock::now() - p0).count();
+            plat.push_back((clock - arr) * 1000.0);
+        }
+        const double span = std::chrono::duration<double>(Clock::now() - p0).count();
+        pp50 = percentile(plat, 0.5);
+        pp99 = percentile(plat, 0.99);
+        plainRate = (double)plainSums / span;
+        ffx_free(dS);
+    }
+    char js[1024];
+    std::snprintf(js, sizeof js,
+                  "{\"mode\": \"serve\", \"requests\": %zu, \"rate_per_s\": %.1f, \"requested_odd_sums\": %llu, "
+                  "\"searched_odd_sums\": %llu, \"batches\": %llu, \"makespan_s\": %.3f, "
+                  "\"served_sums_per_s\": %.1f, \"p50_ms\": %.3f, \"p99_ms\": %.3f, \"map_value\": %llu, "
+                  "\"map_growths\": %llu, \"map_grow_ms\": %.1f, \"solution_lines\": %llu, \"plain_sample\": %u, \"plain_sums_per_s\": %.1f, \"plain_p50_ms\": %.3f, "
+                  "\"plain_p99_ms\": %.3f}\n",
+                  n, opt.rate, (unsigned long long)requested, (unsigned long long)searchedSlots,
+                  (unsigned long long)batches, makespan, (double)requested / makespan, p50, p99,
+                  (unsigned long long)M, (unsigned long long)growths, primeMs, (unsigned long long)lines, opt.plainSample, plainRate, pp50, pp99);
+    std::fputs(js, stdout);
+    if (!opt.jsonOut.empty()) writeFile(opt.jsonOut, js);
+    ffx_free_pinned(mirror);
+    ffx_free(dSlots);
+    ffx_free(dMap);
+    for (int k = 0; k < S; ++k) {
+        ffx_search_plan_destroy(qp[k]);
+        ffx_stream_destroy(st[k]);
+        ffx_event_destroy(ev[k]);
+        ffx_free(dIv[k]);
+        ffx_free_pinned(hIv[k]);
+    }
+    ffx_sieve_plan_destroy(sp);
+    return 0;
+}
+
+}  // namespace ffg

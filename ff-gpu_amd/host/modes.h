@@ -1,0 +1,41 @@
+// ff_sieve's batched (--batch) and served (--serve) modes.
+#ifndef FFG_MODES_H
+#define FFG_MODES_H
+
+#include <string>
+#include <vector>
+
+#include "config.h"
+
+namespace ffg {
+
+struct BatchOptions {
+    std::string rangesFile;      // "sumStart sumLimit" per line, or empty
+    uint64_t seed = 20261203;    // seeded ranges when no file is given
+    uint32_t count = 4096;
+    uint32_t startMax = 1u << 22;
+    double wLo = 8.0, wHi = 16.0;
+    std::string outDir;          // one file per range when set
+    uint32_t reps = 1;           // graph replays timed
+    bool useGraph = true;
+};
+
+struct ServeOptions {
+    uint64_t seed = 20261204;
+    uint32_t count = 20000;
+    uint32_t limLo = 1u << 18, limHi = 1u << 23;
+    double wLo = 6.0, wHi = 18.0;
+    double rate = 2000.0;        // requests per second (Poisson)
+    uint32_t maxBatchRequests = 4096;
+    uint32_t streams = 2;        // batches in flight
+    uint32_t plainSample = 0;    // also time this many requests as sequential plain launches
+    std::string outDir;
+    std::string jsonOut;         // summary line
+};
+
+int runBatch(const Config &cfg, const BatchOptions &opt);
+int runServe(const Config &cfg, const ServeOptions &opt);
+
+}  // namespace ffg
+
+#endif

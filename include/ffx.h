@@ -98,8 +98,20 @@ int ffx_event_destroy(void *ev);
 int ffx_event_record(void *ev, void *stream);
 int ffx_event_wait(void *ev, void *stream);
 int ffx_event_sync(void *ev);
+/* Non-blocking: 0 when the event has completed, 1 when not yet, < 0 on
+ * error (hipEventQuery). */
+int ffx_event_query(void *ev);
 int ffx_event_elapsed_ms(void *start, void *end, double *ms);
 int ffx_device_sync(void);
+
+/* HIP graphs (no reference counterpart): capture the launches submitted to
+ * a created stream, then replay them with one call.  Captured launches read
+ * their arguments as they were at capture time. */
+typedef struct ffx_graph ffx_graph;
+int ffx_graph_begin(void *stream);
+int ffx_graph_end(void *stream, ffx_graph **out);
+int ffx_graph_launch(ffx_graph *graph, void *stream);
+int ffx_graph_destroy(ffx_graph *graph);
 
 /* ---------------------------------------------------------------------- */
 /* sieve                                                                    */
@@ -123,6 +135,13 @@ int ffx_sieve_plan_create(ffx_sieve_plan **out, uint64_t maxPrimeMapValue);
 int ffx_sieve_plan_create_from_primes(ffx_sieve_plan **out, const uint32_t *h_primes, uint32_t numPrimes,
                                       uint64_t maxPrimeMapValue);
 int ffx_sieve_plan_destroy(ffx_sieve_plan *plan);
+
+/* Tunable sieve parameters (no reference counterpart; the reference fixes
+ * its kernel geometry at compile time, source/sieve_slab_kernel.h:40-67). */
+#define FFX_SIEVE_SEGMENT_BYTES 1  /* LDS tile: map bytes per segment, 32768 or 65536 */
+#define FFX_SIEVE_BLOCKS_PER_CU 2  /* persistent grid: 1024-thread workgroups per CU (1..8) */
+int ffx_sieve_plan_set(ffx_sieve_plan *plan, int param, uint32_t value);
+int ffx_sieve_plan_get(const ffx_sieve_plan *plan, int param, uint32_t *value);
 
 /* Replaces SievePoolOps::slabComputeAsync (source/sieve_slab_engine.h:93-97):
  * writes the map bytes covering values [segLo, segHi) into d_slab, byte i of
@@ -179,6 +198,19 @@ typedef struct ffx_gpu_record {
 
 typedef struct ffx_search_plan ffx_search_plan;
 
+/* One interval of a batched search: the odd sums first, first+2, ..., last
+ * fill slots slotOffset, slotOffset+1, ... of the slot array. */
+typedef struct ffx_interval {
+    uint32_t first;
+    uint32_t last;
+    uint64_t slotOffset;
+} ffx_interval;
+
+/* Tunable search parameters (ffx_search_plan_set / _get). */
+#define FFX_SEARCH_GRAB 1           /* sums a wavefront takes per atomic (1..64) */
+#define FFX_SEARCH_EAGER_SPLITS 2   /* splits per sum resolved eagerly before batching candidates */
+#define FFX_SEARCH_BLOCKS_PER_CU 3  /* persistent grid: workgroups of 4 wavefronts per CU (1..16) */
+
 /* Prepares the search for sums up to sumLimitMax: uploads the smallest-
  * prime-factor table used to factor terms (< sumLimitMax) and allocates the
  * plan's counters and compaction scratch. */
@@ -191,6 +223,29 @@ int ffx_search_plan_destroy(ffx_search_plan *plan);
  * ordered (one stream) because they share its work counter. */
 int ffx_search_slots_async(ffx_search_plan *plan, const void *d_map, uint64_t maxPrimeMapValue, uint32_t sumStart,
                            uint32_t sumLimit, uint32_t *d_slots, void *stream);
+
+/* Batched search (no reference counterpart; the reference launches one
+ * range per call): every odd sum of nintervals disjoint intervals (device
+ * array, slotOffset ascending, sums ascending) in ONE persistent launch;
+ * wavefronts pull sums from a shared counter, largest first, so sub-ranges
+ * of very different cost balance across the device.  lastSum is the
+ * largest sum of any interval (checked against the plan). */
+int ffx_search_intervals_async(ffx_search_plan *plan, const void *d_map, uint64_t maxPrimeMapValue,
+                               const ffx_interval *d_intervals, uint32_t nintervals, uint64_t totalSlots,
+                               uint32_t lastSum, uint32_t *d_slots, void *stream);
+
+/* Same, but the slot of sum s is d_slots[(s - slotSumBase) / 2] whatever
+ * interval holds it: one resident slot array for all sums of a stream of
+ * batches (slotSumBase odd, <= every interval's first sum).  Slots at or
+ * beyond slotCapacity are never written. */
+int ffx_search_intervals_by_sum_async(ffx_search_plan *plan, const void *d_map, uint64_t maxPrimeMapValue,
+                                      const ffx_interval *d_intervals, uint32_t nintervals, uint64_t totalSlots,
+                                      uint32_t lastSum, uint32_t slotSumBase, uint32_t *d_slots, uint64_t slotCapacity,
+                                      void *stream);
+
+/* Tuning knobs of a plan (FFX_SEARCH_*); values out of range fail. */
+int ffx_search_plan_set(ffx_search_plan *plan, int param, uint32_t value);
+int ffx_search_plan_get(const ffx_search_plan *plan, int param, uint32_t *value);
 
 /* Ordered compaction of slots into solutions: d_out[0..*d_count) in slot
  * order.  d_out must hold one entry per solved slot (nslots is an upper

@@ -1,0 +1,127 @@
+"""Batched (--batch) and served (--serve) ff_sieve on the GPU.
+
+Every range of a batch and every request of a served stream must print
+exactly what a plain `ff_sieve <sumStart> <sumLimit>` (and ff_seg) prints for
+it alone: checked against the oracle (oracle/ffref, ff_seg-identical stdout)
+on small sums, and against plain GPU runs on the full-size batched workload
+(4096 seeded ranges, starts below 2^22, widths 2^8..2^16)."""
+import json
+import os
+import random
+import subprocess
+import sys
+import tempfile
+
+import pytest
+
+import ffxpy
+from conftest import normalize, run_ffref
+
+pytestmark = pytest.mark.gpu
+
+
+def ff_sieve(*args, timeout=300):
+    print("[case] ff_sieve", *args, flush=True)
+    sys.stdout.flush()
+    return subprocess.run([ffxpy.FF_SIEVE] + [str(a) for a in args], capture_output=True, timeout=timeout)
+
+
+def summary(p):
+    assert p.returncode == 0, p.stderr.decode()[-3000:]
+    return json.loads(p.stdout.decode().strip().splitlines()[-1])
+
+
+def oracle_text(a, b):
+    p = run_ffref([a, b], timeout=120)
+    assert p.returncode == 0
+    return normalize(p.stdout.decode())
+
+
+EDGE_RANGES = [(5, 5), (4, 6), (1, 100), (100, 50), (7, 2627), (2, 9), (1000, 1003), (65535, 65535),
+               (65536, 65600), (60001, 70000), (60001, 70000), (61001, 65003), (17, 17), (131000, 131071)]
+
+
+def seeded_ranges(seed, n, top):
+    rng = random.Random(seed)
+    out = []
+    for _ in range(n):
+        a = rng.randrange(5, top)
+        w = int(2 ** rng.uniform(2, 13))
+        out.append((a, min(top, a + w - 1)))
+    return out
+
+
+@pytest.mark.parametrize("graph", [1, 0])
+def test_batch_ranges_match_oracle(graph):
+    ranges = EDGE_RANGES + seeded_ranges(11, 34, 1 << 17)
+    with tempfile.TemporaryDirectory() as d:
+        rf = os.path.join(d, "ranges.txt")
+        with open(rf, "w") as f:
+            f.writelines(f"{a} {b}\n" for a, b in ranges)
+        s = summary(ff_sieve("--batch", "--opt", f"ranges={rf}", "--opt", f"out_dir={d}", "--opt", f"graph={graph}",
+                             "--opt", "reps=2"))
+        assert s["ranges"] == len(ranges) and s["graph"] == bool(graph)
+        lines = 0
+        for i, (a, b) in enumerate(ranges):
+            got = normalize(open(os.path.join(d, f"range_{i:05d}.txt")).read())
+            assert got == oracle_text(a, b), (i, a, b)
+            lines += got.count(") sum =")
+        assert s["solution_lines"] == lines
+
+
+def test_batch_c4_ranges_match_plain_runs():
+    # the full batched workload; a sample of its ranges (the largest sums
+    # among them) against plain single-range GPU runs
+    with tempfile.TemporaryDirectory() as d:
+        s = summary(ff_sieve("--batch", "--opt", f"out_dir={d}", timeout=600))
+        assert s["ranges"] == 4096 and s["graph"] is True
+        ranges = [tuple(map(int, l.split())) for l in subprocess.run(
+            [os.path.join(os.path.dirname(ffxpy.FF_SIEVE), "ff_hostcheck"), "batch", "20261203", "4096"],
+            capture_output=True, text=True, check=True).stdout.splitlines()]
+        assert len(ranges) == 4096
+        order = sorted(range(4096), key=lambda i: -ranges[i][1])
+        pick = order[:3] + random.Random(5).sample(order[3:], 5)
+        for i in pick:
+            a, b = ranges[i]
+            plain = ff_sieve(a, b)
+            assert plain.returncode == 0
+            got = normalize(open(os.path.join(d, f"range_{i:05d}.txt")).read())
+            assert got == normalize(plain.stdout.decode()), (i, a, b)
+
+
+def test_serve_requests_match_oracle_alone():
+    # each request's output in the mix (continuous batching over two
+    # streams, shared result slots) equals its output alone
+    count = 60
+    with tempfile.TemporaryDirectory() as d:
+        s = summary(ff_sieve("--serve", "--opt", f"count={count}", "--opt", "seed=7", "--opt", "lim_lo=1024",
+                             "--opt", "lim_hi=131071", "--opt", "wlo=2", "--opt", "whi=13", "--opt", "rate=3000",
+                             "--opt", "max_batch=8", "--opt", "streams=2", "--opt", f"out_dir={d}"))
+        assert s["requests"] == count and s["batches"] >= 2
+        reqs = [tuple(map(int, l.split()[:2])) for l in subprocess.run(
+            [os.path.join(os.path.dirname(ffxpy.FF_SIEVE), "ff_hostcheck"), "served", "7", str(count), "3000",
+             "1024", "131071", "2", "13"], capture_output=True, text=True, check=True).stdout.splitlines()]
+        assert len(reqs) == count
+        for i, (a, b) in enumerate(reqs):
+            got = normalize(open(os.path.join(d, f"request_{i:05d}.txt")).read())
+            assert got == oracle_text(a, b), (i, a, b)
+
+
+def test_serve_mix_matches_plain_on_large_sums():
+    # requests up to sumLimit 2^23 (capped PRIMEMEM headers included), a
+    # device map bounded by --map-cap; each against a plain run
+    count = 24
+    with tempfile.TemporaryDirectory() as d:
+        s = summary(ff_sieve("--map-cap", 2 ** 34, "--serve", "--opt", f"count={count}", "--opt", "seed=9",
+                             "--opt", "wlo=6", "--opt", "whi=12", "--opt", "rate=500", "--opt", f"out_dir={d}",
+                             timeout=600))
+        assert s["requests"] == count
+        reqs = [tuple(map(int, l.split()[:2])) for l in subprocess.run(
+            [os.path.join(os.path.dirname(ffxpy.FF_SIEVE), "ff_hostcheck"), "served", "9", str(count), "500",
+             str(1 << 18), str(1 << 23), "6", "12"], capture_output=True, text=True, check=True).stdout.splitlines()]
+        for i in random.Random(3).sample(range(count), 6):
+            a, b = reqs[i]
+            plain = ff_sieve("--map-cap", 2 ** 34, a, b, timeout=600)
+            assert plain.returncode == 0
+            got = normalize(open(os.path.join(d, f"request_{i:05d}.txt")).read())
+            assert got == normalize(plain.stdout.decode()), (i, a, b)

@@ -161,3 +161,66 @@ def test_workloads_are_seeded_and_shaped():
     for a_, b_, _ in r:
         a_, b_ = int(a_), int(b_)
         assert (1 << 18) <= b_ <= (1 << 23) and a_ >= 5 and b_ - a_ + 1 <= (1 << 18)
+
+
+def _spans_bruteforce(spans, covered, top):
+    a = set()
+    for lo, hi in spans:
+        a.update(range(lo, hi + 1, 2))
+    c = set()
+    for lo, hi in covered:
+        c.update(range(lo, hi + 1, 2))
+    return a, a - c
+
+
+def _runs(sums):
+    """Sorted odd sums -> maximal [lo, hi] runs of consecutive odd sums."""
+    out = []
+    for s in sorted(sums):
+        if out and s == out[-1][1] + 2:
+            out[-1][1] = s
+        else:
+            out.append([s, s])
+    return [tuple(x) for x in out]
+
+
+@pytest.mark.parametrize("rng_seed", [3, 4])
+def test_interval_union_subtract_and_slots(rng_seed):
+    # the batched/served modes share one slot per distinct odd sum: the
+    # union, its slot offsets and the still-missing spans against sets
+    rng = random.Random(rng_seed)
+    for case in range(40):
+        def span():
+            lo = 2 * rng.randrange(2, 400) + 1
+            return lo, lo + 2 * rng.randrange(0, 60 if case % 3 else 2)
+        spans = [span() for _ in range(rng.randrange(1, 30))]
+        covered = [span() for _ in range(rng.randrange(0, 20))]
+        top = max(h for _, h in spans)
+        feed = f"{len(spans)} {len(covered)}\n" + "".join(f"{a} {b}\n" for a, b in spans + covered)
+        out = subprocess.run([HC, "spans"], input=feed, capture_output=True, text=True, check=True).stdout
+        u_txt, m_txt, s_txt = out.split("--\n")
+        allsums, missing = _spans_bruteforce(spans, covered, top)
+        u = [tuple(map(int, l.split())) for l in u_txt.splitlines()]
+        assert [(a, b) for a, b, _ in u] == _runs(allsums)
+        off = 0
+        for a, b, o in u:
+            assert o == off
+            off += (b - a) // 2 + 1
+        assert [tuple(map(int, l.split())) for l in m_txt.splitlines()] == _runs(missing)
+        order = sorted(allsums)
+        slots = [int(x) for x in s_txt.split()]
+        for i, s in enumerate(range(1, top + 7, 2)):
+            assert slots[i] == (order.index(s) if s in allsums else -1), (case, s)
+
+
+@pytest.mark.parametrize("a,b", [(4, 6), (5, 5), (1, 100), (100, 50), (7, 2627), (65535, 65535),
+                                 (65536, 65600), (1000, 1003), (2, 9)])
+def test_job_span_matches_oracle_chunks(ffref, a, b):
+    # the odd sums a batched range evaluates are exactly those of ff_seg's
+    # chunks (sumLimit+1 and skipped sums of tiny ranges included)
+    j = kv("job", a, b)
+    chunks = [tuple(map(int, c.split())) for c in (j["chunk"] if isinstance(j["chunk"], list) else [j["chunk"]])]
+    ref = [tuple(map(int, l.split())) for l in run_ffref(["--chunks", a, b]).stdout.decode().splitlines()]
+    assert chunks == ref
+    odd = [s for lo, hi in ref for s in range(lo | 1, hi + 1, 2)]
+    assert j["any"] == "1" and tuple(map(int, j["span"].split())) == (min(odd), max(odd))
