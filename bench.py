@@ -93,6 +93,19 @@ def cpu_baseline(sum_limit, timeout_s):
     }
 
 
+def calibrate_copy(nbytes):
+    """A plain device-to-device copy of known size (torch's copy kernel),
+    three times: FETCH_SIZE and WRITE_SIZE of one dispatch against nbytes
+    give the counters' scale."""
+    import torch
+    src = torch.ones(nbytes, dtype=torch.uint8, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibrate_copy_bytes": nbytes, "kib": nbytes / 1024}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,7 +114,12 @@ def main():
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-timeout", type=float, default=150.0)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only copy a BYTES device buffer three times: the run that scales the HBM byte counters")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the sieve and search are single-device here (each GPU would "
               "re-sieve locally and search a disjoint sum range); run with --gpus 1", file=sys.stderr)
@@ -205,7 +223,7 @@ def main():
         "dtype": "u64",
         "data": "synthetic: the workload is the sum range itself (no input data)",
         "config": {"workload": desc, "sumStart": 5, "sumLimit": sum_limit, "odd_sums": nodd,
-                   "maxPrimeMapValue": M, "map_bytes": map_bytes, "device": dev.name.decode(),
+                   "maxPrimeMapValue": M, "map_bytes": map_bytes, "device": dev.name.decode() or dev.arch.decode(),
                    "cus": dev.multiProcessorCount},
         "roofline": roof,
         "roofline_sieve": roof_sieve,

@@ -49,3 +49,23 @@ def test_cli_fails_loudly_on_bad_flags():
     assert p.returncode == 1 and "unknown option" in p.stderr
     p = subprocess.run([ffxpy.FF_SIEVE, "--no-gpu", "5", "100"], capture_output=True, text=True)
     assert p.returncode == 1
+
+
+REF_HEADERS = "/root/reference/source"
+
+
+def test_reference_adapter_compiles_against_reference_headers():
+    # the reference-side adapter type-checks against the reference's own
+    # SievePoolOps / GpuRecord declarations and exports the three getters
+    import pytest
+    if not os.path.exists(os.path.join(REF_HEADERS, "sieve_slab_engine.h")):
+        pytest.skip("reference headers absent")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src = os.path.join(root, "ff-gpu_amd", "bindings", "reference_adapter.cpp")
+    p = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Wextra", "-Werror",
+                        "-I" + os.path.join(root, "include"), "-I" + REF_HEADERS, src], capture_output=True, text=True)
+    assert p.returncode == 0, p.stderr
+    so = os.path.join(os.path.dirname(ffxpy.LIB_PATH), "libffx_refadapter.so")
+    out = subprocess.run(["nm", "-D", "--defined-only", so], capture_output=True, text=True).stdout
+    for n in ("SievePoolGet_gfx950", "SieveSlabEngineGetLaunchFn_gfx950", "SearchKernelGetLaunchFn_gfx950"):
+        assert n in out, n
