@@ -264,7 +264,7 @@ bool modeOptions(const Config &cfg, BatchOptions *b, ServeOptions *sv)
         else if (k == "lim_lo") sv->limLo = (uint32_t)u;
         else if (k == "lim_hi") sv->limHi = (uint32_t)u;
         else if (k == "rate") sv->rate = d;
-        else if (k == "max_batch") sv->maxBatchRequests = (uint32_t)u;
+        else if (k == "batch_sums") sv->batchSums = u;
         else if (k == "streams") sv->streams = (uint32_t)u;
         else if (k == "plain_sample") sv->plainSample = (uint32_t)u;
         else if (k == "json") sv->jsonOut = v;

@@ -351,19 +351,28 @@ struct InFlight {
 
 }  // namespace
 
+// Map bound that keeps the probes of sums up to L in the bitmap as far as
+// ff_seg's own geometry does (uncapped: ((L+1)>>1)^2/4 + 2, rounded up to a
+// whole map byte, segmentedSieve.C:874-879).
+static uint64_t uncappedMapValue(uint32_t L)
+{
+    const uint64_t mgp = ((uint64_t)L + 1) >> 1;
+    const uint64_t need = mgp * mgp / 4 + 2;
+    return ((need + 15) & ~15ull) - 1;
+}
+
 int runServe(const Config &cfg, const ServeOptions &opt)
 {
     const std::vector<Request> reqs =
         servedRequests(opt.seed, opt.count, opt.limLo, opt.limHi, opt.wLo, opt.wHi, opt.rate);
     const size_t n = reqs.size();
     std::vector<RangeJob> jobs(n);
-    uint64_t maxM = 15, requested = 0;
+    uint64_t requested = 0;
     uint32_t maxSum = 5;
     for (size_t i = 0; i < n; ++i) {
         jobs[i] = makeJob(reqs[i].sumStart, reqs[i].sumLimit, cfg.primeMem);
         const RangeJob &j = jobs[i];
         if (j.geom.fatal || !j.any) continue;
-        maxM = std::max(maxM, j.geom.maxPrimeMapValue);
         maxSum = std::max(maxSum, j.hi);
         requested += (j.hi - j.lo) / 2 + 1;
     }
@@ -377,23 +386,27 @@ int runServe(const Config &cfg, const ServeOptions &opt)
     const uint32_t base = 5;
     const uint64_t nslots = (maxSum - base) / 2 + 1;
     const uint32_t ivCap = 1u << 16;  // intervals per batch
+    const uint64_t batchSums = std::max<uint64_t>(64, opt.batchSums);
     const uint64_t fixedBytes = nslots * 4 + (uint64_t)S * ivCap * sizeof(ffx_interval);
-    // admission against the device budget (source/budget.cpp:15-42): the
-    // resident map gets what the backing holds after the fixed buffers
-    uint64_t M = mapValueFor(maxM, cfg.mapCap);
+    // Admission against the device budget (source/budget.cpp:15-42): the
+    // resident map gets what the backing holds after the fixed buffers.
+    // Probe values above the map are answered by Miller-Rabin, so the bound
+    // only trades speed for memory.
     if (dev.budget.backing < fixedBytes + (64ull << 20)) {
         std::fprintf(stderr, "[ff_sieve] ERROR: budget %s cannot hold the serving buffers\n",
                      bytesToHuman(dev.budget.backing).c_str());
         return 1;
     }
     const uint64_t mapRoom = dev.budget.backing - fixedBytes;
+    uint64_t M = mapValueFor(uncappedMapValue(maxSum), cfg.mapCap);
     if ((M >> 4) + 1 > mapRoom) M = mapValueFor(((mapRoom - 1) << 4) - 1, 0);
     const uint64_t mapBytesMax = (M >> 4) + 1;
     std::fprintf(stderr,
-                 "[ff_sieve] serve: %zu requests, %llu requested odd sums up to %u; resident map up to M=%llu (%s, "
-                 "grown on demand), budget backing %s, %d streams\n",
-                 n, (unsigned long long)requested, maxSum, (unsigned long long)M, bytesToHuman(mapBytesMax).c_str(),
-                 bytesToHuman(dev.budget.backing).c_str(), S);
+                 "[ff_sieve] serve: %zu requests at %.0f/s, %llu requested odd sums up to %u; resident map up to "
+                 "M=%llu (%s, grown on demand), budget backing %s, %d streams, %llu sums per batch\n",
+                 n, opt.rate, (unsigned long long)requested, maxSum, (unsigned long long)M,
+                 bytesToHuman(mapBytesMax).c_str(), bytesToHuman(dev.budget.backing).c_str(), S,
+                 (unsigned long long)batchSums);
 
     int rc;
     ffx_sieve_plan *sp = nullptr;
@@ -410,20 +423,30 @@ int runServe(const Config &cfg, const ServeOptions &opt)
         hIv[k] = (ffx_interval *)ffx_alloc_pinned((uint64_t)ivCap * sizeof(ffx_interval));
         if (!st[k] || !ev[k] || !dIv[k] || !hIv[k]) return fail("stream resources", FFX_ENOMEM);
     }
+    void *gst = ffx_stream_create(), *gev = ffx_event_create();
     void *dMap = ffx_alloc((mapBytesMax + 255) & ~255ull);
     uint32_t *dSlots = (uint32_t *)ffx_alloc(nslots * 4);
     uint32_t *mirror = (uint32_t *)ffx_alloc_pinned(nslots * 4);
-    if (!dMap || !dSlots || !mirror) return fail("allocation", FFX_ENOMEM);
+    if (!gst || !gev || !dMap || !dSlots || !mirror) return fail("allocation", FFX_ENOMEM);
+
+    // The first GiB of the map is sieved before the stream opens (small
+    // values are always probed); the rest grows on its own stream while
+    // batches run, each batch using the bytes finished when it launches.
+    const uint64_t growStep = 16ull << 30;
+    uint64_t mapBytesDone = std::min<uint64_t>(mapBytesMax, 1ull << 30), growTarget = 0, growths = 1;
+    if ((rc = ffx_sieve_map_range_async(sp, dMap, 0, mapBytesDone, gst)) || (rc = ffx_stream_sync(gst)))
+        return fail("map", rc);
+    bool growing = false;
+    uint64_t wantBytes = mapBytesDone;
 
     std::vector<double> doneAt(n, -1.0);
-    std::vector<char> admitted(n, 0);
-    std::vector<size_t> waiting;  // admitted, output not yet started
-    Spans computed, inflightSpans;
+    Spans computed, claimed;  // claimed: computed, running or queued
+    std::deque<std::pair<uint32_t, uint32_t>> work;  // queued spans, arrival order
     std::deque<InFlight> inflight;
     std::vector<bool> busy(S, false);
-    uint64_t mapBytesDone = 0, batches = 0, searchedSlots = 0, lines = 0, growths = 0;
+    std::vector<size_t> waiting;
+    uint64_t batches = 0, searchedSlots = 0, lines = 0;
     std::mutex linesMu;
-    double primeMs = 0;
     const bool writeOut = !opt.outDir.empty();
     if (writeOut) mkdir(opt.outDir.c_str(), 0755);
 
@@ -435,8 +458,7 @@ int runServe(const Config &cfg, const ServeOptions &opt)
         uint64_t nl = 0;
         if (jobs[id].geom.fatal) out = fatalText(jobs[id]);
         else
-            nl = emitJob(out, jobs[id], [&](uint32_t s) { return mirror[(s - base) >> 1]; },
-                         (uint64_t)(primeMs * 1000), 0);
+            nl = emitJob(out, jobs[id], [&](uint32_t s) { return mirror[(s - base) >> 1]; }, 0, 0);
         if (writeOut) {
             char name[64];
             std::snprintf(name, sizeof name, "/request_%05zu.txt", id);
@@ -448,67 +470,65 @@ int runServe(const Config &cfg, const ServeOptions &opt)
     });
 
     size_t next = 0, completed = 0;
-    std::deque<size_t> pending;
     while (completed < n) {
-        bool progress = false;
-        const double t = now();
-        while (next < n && reqs[next].arrivalS <= t) pending.push_back(next++);
-        // launch batches while a stream is free
-        while (!pending.empty()) {
-            int k = -1;
-            for (int i = 0; i < S; ++i)
-                if (!busy[i]) {
-                    k = i;
-                    break;
-                }
-            if (k < 0) break;
-            Spans want;
-            uint64_t needM = 15;
-            size_t taken = 0;
-            while (!pending.empty() && taken < opt.maxBatchRequests) {
-                const size_t id = pending.front();
-                pending.pop_front();
-                ++taken;
-                admitted[id] = 1;
-                waiting.push_back(id);
-                const RangeJob &j = jobs[id];
-                if (j.geom.fatal || !j.any) continue;
-                want.push_back({j.lo, j.hi});
-                needM = std::max(needM, std::min(j.geom.maxPrimeMapValue, M));
+        bool progress = false, admitted = false;
+        // admission: arrived requests queue the sums nothing has claimed
+        for (const double t = now(); next < n && reqs[next].arrivalS <= t; ++next) {
+            const RangeJob &j = jobs[next];
+            progress = admitted = true;
+            if (j.geom.fatal || !j.any) {
+                pool.push(next);
+                ++completed;
+                continue;
             }
-            Spans wantU;
-            mergeInto(wantU, want);
-            Spans missing = subtract(subtract(wantU, computed), inflightSpans);
-            progress = true;
-            if (missing.empty()) continue;
-            // the batch's intervals, in slot order
+            const Spans missing = subtract(Spans{{j.lo, j.hi}}, claimed);
+            for (const auto &m : missing) work.push_back(m);
+            mergeInto(claimed, missing);
+            waiting.push_back(next);
+            wantBytes = std::max(wantBytes, (std::min(M, uncappedMapValue(j.hi)) >> 4) + 1);
+        }
+        // map growth, concurrent with the batches
+        if (growing) {
+            const int q = ffx_event_query(gev);
+            if (q < 0) return fail("map growth", q);
+            if (q == 0) {
+                mapBytesDone = growTarget;
+                growing = false;
+                progress = true;
+            }
+        }
+        if (!growing && wantBytes > mapBytesDone) {
+            growTarget = std::min(wantBytes, mapBytesDone + growStep);
+            if ((rc = ffx_sieve_map_range_async(sp, dMap, mapBytesDone, growTarget, gst)) ||
+                (rc = ffx_event_record(gev, gst)))
+                return fail("map growth", rc);
+            growing = true;
+            ++growths;
+        }
+        // batches: the oldest queued sums, at most batchSums per launch
+        for (int k = 0; k < S && !work.empty(); ++k) {
+            if (busy[k]) continue;
             std::vector<ffx_interval> iv;
             uint64_t total = 0;
-            for (const auto &m : missing) {
+            while (!work.empty() && total < batchSums && iv.size() < ivCap) {
+                auto &f = work.front();
+                const uint64_t cnt = (f.second - f.first) / 2 + 1, take = std::min(cnt, batchSums - total);
                 ffx_interval x;
-                x.first = m.first;
-                x.last = m.second;
-                x.slotOffset = total;
-                total += (m.second - m.first) / 2 + 1;
+                x.first = f.first;
+                x.last = f.first + 2 * (uint32_t)(take - 1);
+                x.slotOffset = 0;
                 iv.push_back(x);
+                total += take;
+                if (take == cnt) work.pop_front();
+                else f.first = x.last + 2;
             }
-            if (iv.size() > ivCap) {
-                std::fprintf(stderr, "[ff_sieve] error: %zu intervals in one batch (max %u)\n", iv.size(), ivCap);
-                return 1;
-            }
-            // grow the resident map to this batch's largest geometry
-            // (whole GiB steps; the host waits for the growth, so batches
-            // later launched on other streams see the new bytes)
-            uint64_t needBytes = (needM >> 4) + 1;
-            if (needBytes > mapBytesDone) {
-                needBytes = std::min<uint64_t>(mapBytesMax, (needBytes + (1ull << 30) - 1) & ~((1ull << 30) - 1));
-                const auto tg = Clock::now();
-                if ((rc = ffx_sieve_map_range_async(sp, dMap, mapBytesDone, needBytes, st[k])) ||
-                    (rc = ffx_stream_sync(st[k])))
-                    return fail("map growth", rc);
-                mapBytesDone = needBytes;
-                primeMs += msSince(tg);
-                ++growths;
+            std::sort(iv.begin(), iv.end(), [](const ffx_interval &a, const ffx_interval &b) { return a.first < b.first; });
+            Spans spans;
+            uint64_t off = 0;
+            for (ffx_interval &x : iv) {
+                x.slotOffset = off;
+                off += (x.last - x.first) / 2 + 1;
+                spans.push_back({x.first, x.last});
             }
             const uint64_t Muse = std::min(M, (mapBytesDone << 4) - 1);
             std::memcpy(hIv[k], iv.data(), iv.size() * sizeof(ffx_interval));
@@ -518,17 +538,20 @@ int runServe(const Config &cfg, const ServeOptions &opt)
                                                         iv.back().last, base, dSlots, nslots, st[k])))
                 return fail("served search", rc);
             for (const ffx_interval &x : iv) {
-                const uint64_t off = (x.first - base) >> 1, cnt = (x.last - x.first) / 2 + 1;
-                if ((rc = ffx_copy_d2h_async(mirror + off, dSlots + off, cnt * 4, st[k]))) return fail("slot copy", rc);
+                const uint64_t o = (x.first - base) >> 1, cnt = (x.last - x.first) / 2 + 1;
+                if ((rc = ffx_copy_d2h_async(mirror + o, dSlots + o, cnt * 4, st[k]))) return fail("slot copy", rc);
             }
             if ((rc = ffx_event_record(ev[k], st[k]))) return fail("event", rc);
             busy[k] = true;
-            inflight.push_back({k, missing});
-            mergeInto(inflightSpans, missing);
+            InFlight f;
+            f.stream = k;
+            f.spans = std::move(spans);
+            inflight.push_back(std::move(f));
             ++batches;
             searchedSlots += total;
+            progress = true;
         }
-        // completed batches
+        // finished batches
         bool landed = false;
         for (auto it = inflight.begin(); it != inflight.end();) {
             const int q = ffx_event_query(ev[it->stream]);
@@ -542,36 +565,35 @@ int runServe(const Config &cfg, const ServeOptions &opt)
                 ++it;
             }
         }
-        if (landed) {
-            inflightSpans.clear();
-            for (const InFlight &f : inflight) mergeInto(inflightSpans, f.spans);
+        // requests whose sums are all in the mirror go to the formatting pool
+        if (landed || admitted) {
             progress = true;
-        }
-        // requests whose sums are all computed go to the formatting pool
-        for (size_t i = 0; i < waiting.size();) {
-            const size_t id = waiting[i];
-            const RangeJob &j = jobs[id];
-            if (j.geom.fatal || !j.any || covers(computed, j.lo, j.hi)) {
-                pool.push(id);
-                ++completed;
-                waiting[i] = waiting.back();
-                waiting.pop_back();
-                progress = true;
-            } else {
-                ++i;
+            for (size_t i = 0; i < waiting.size();) {
+                const size_t id = waiting[i];
+                if (covers(computed, jobs[id].lo, jobs[id].hi)) {
+                    pool.push(id);
+                    ++completed;
+                    waiting[i] = waiting.back();
+                    waiting.pop_back();
+                } else {
+                    ++i;
+                }
             }
         }
         if (!progress) std::this_thread::sleep_for(std::chrono::microseconds(20));
     }
     pool.finish();
     const double makespan = now();
+    if (growing && (rc = ffx_event_sync(gev))) return fail("map growth", rc);
+    if (growing) mapBytesDone = growTarget;
     std::vector<double> lat(n);
     for (size_t i = 0; i < n; ++i) lat[i] = (doneAt[i] - reqs[i].arrivalS) * 1000.0;
     const double p50 = percentile(lat, 0.50), p99 = percentile(lat, 0.99);
 
-    // sequential plain launches of a sample, same arrival times, FIFO, one
-    // request at a time, no sharing between requests, map resident
-    double pp50 = 0, pp99 = 0, plainRate = 0;
+    // Sequential plain launches of the first plainSample requests at their
+    // arrival times: FIFO, one request at a time, nothing shared between
+    // requests, the map resident at its full size.
+    double pp50 = 0, pp99 = 0, plainRate = 0, plainSpan = 0;
     uint64_t plainSums = 0;
     if (opt.plainSample) {
         const size_t m = std::min<size_t>(opt.plainSample, n);
@@ -579,12 +601,13 @@ int runServe(const Config &cfg, const ServeOptions &opt)
         uint32_t *dS = (uint32_t *)ffx_alloc(nslots * 4);
         if (!dS) return fail("plain slots", FFX_ENOMEM);
         if (mapBytesDone < mapBytesMax) {
-            if ((rc = ffx_sieve_map_range_async(sp, dMap, mapBytesDone, mapBytesMax, st[0]))) return fail("map", rc);
-            ffx_stream_sync(st[0]);
+            if ((rc = ffx_sieve_map_range_async(sp, dMap, mapBytesDone, mapBytesMax, gst)) ||
+                (rc = ffx_stream_sync(gst)))
+                return fail("map", rc);
             mapBytesDone = mapBytesMax;
         }
+        std::vector<uint32_t> hostSlots(nslots);
         const auto p0 = Clock::now();
-        double clock = 0;  // seconds since the sample's first arrival
         const double a0 = reqs[0].arrivalS;
         for (size_t i = 0; i < m; ++i) {
             const RangeJob &j = jobs[i];
@@ -598,41 +621,46 @@ int runServe(const Config &cfg, const ServeOptions &opt)
                 x.slotOffset = 0;
                 const uint64_t cnt = (j.hi - j.lo) / 2 + 1;
                 std::memcpy(hIv[0], &x, sizeof x);
-                ffx_copy_h2d_async(dIv[0], hIv[0], sizeof x, st[0]);
-                if ((rc = ffx_search_intervals_by_sum_async(qp[0], dMap, M, dIv[0], 1, cnt, j.hi, base, dS, nslots,
+                if ((rc = ffx_copy_h2d_async(dIv[0], hIv[0], sizeof x, st[0])) ||
+                    (rc = ffx_search_intervals_by_sum_async(qp[0], dMap, M, dIv[0], 1, cnt, j.hi, base, dS, nslots,
                                                             st[0])))
                     return fail("plain search", rc);
-                const uint64_t off = (j.lo - base) >> 1;
-                ffx_copy_d2h_async(mirror + off, dS + off, cnt * 4, st[0]);
-                ffx_stream_sync(st[0]);
+                const uint64_t o = (j.lo - base) >> 1;
+                if ((rc = ffx_copy_d2h_async(hostSlots.data() + o, dS + o, cnt * 4, st[0])) ||
+                    (rc = ffx_stream_sync(st[0])))
+                    return fail("plain copy", rc);
                 plainSums += cnt;
             }
             std::string out;
-            if (!j.geom.fatal) emitJob(out, j, [&](uint32_t s) { return mirror[(s - base) >> 1]; }, 0, 0);
-            clock = std::chrono::duration<double>(Clock::now() - p0).count();
-            plat.push_back((clock - arr) * 1000.0);
+            if (!j.geom.fatal) emitJob(out, j, [&](uint32_t s) { return hostSlots[(s - base) >> 1]; }, 0, 0);
+            plat.push_back((std::chrono::duration<double>(Clock::now() - p0).count() - arr) * 1000.0);
         }
-        const double span = std::chrono::duration<double>(Clock::now() - p0).count();
+        plainSpan = std::chrono::duration<double>(Clock::now() - p0).count();
         pp50 = percentile(plat, 0.5);
         pp99 = percentile(plat, 0.99);
-        plainRate = (double)plainSums / span;
+        plainRate = (double)plainSums / plainSpan;
         ffx_free(dS);
     }
-    char js[1024];
+    char js[1536];
     std::snprintf(js, sizeof js,
                   "{\"mode\": \"serve\", \"requests\": %zu, \"rate_per_s\": %.1f, \"requested_odd_sums\": %llu, "
-                  "\"searched_odd_sums\": %llu, \"batches\": %llu, \"makespan_s\": %.3f, "
-                  "\"served_sums_per_s\": %.1f, \"p50_ms\": %.3f, \"p99_ms\": %.3f, \"map_value\": %llu, "
-                  "\"map_growths\": %llu, \"map_grow_ms\": %.1f, \"solution_lines\": %llu, \"plain_sample\": %u, \"plain_sums_per_s\": %.1f, \"plain_p50_ms\": %.3f, "
-                  "\"plain_p99_ms\": %.3f}\n",
+                  "\"searched_odd_sums\": %llu, \"batches\": %llu, \"batch_sums\": %llu, \"streams\": %d, "
+                  "\"makespan_s\": %.3f, \"served_sums_per_s\": %.1f, \"searched_sums_per_s\": %.1f, "
+                  "\"p50_ms\": %.3f, \"p99_ms\": %.3f, \"map_value\": %llu, \"map_bytes\": %llu, "
+                  "\"map_growths\": %llu, \"solution_lines\": %llu, \"plain_sample\": %u, \"plain_span_s\": %.3f, "
+                  "\"plain_sums_per_s\": %.1f, \"plain_p50_ms\": %.3f, \"plain_p99_ms\": %.3f}\n",
                   n, opt.rate, (unsigned long long)requested, (unsigned long long)searchedSlots,
-                  (unsigned long long)batches, makespan, (double)requested / makespan, p50, p99,
-                  (unsigned long long)M, (unsigned long long)growths, primeMs, (unsigned long long)lines, opt.plainSample, plainRate, pp50, pp99);
+                  (unsigned long long)batches, (unsigned long long)batchSums, S, makespan,
+                  (double)requested / makespan, (double)searchedSlots / makespan, p50, p99, (unsigned long long)M,
+                  (unsigned long long)mapBytesMax, (unsigned long long)growths, (unsigned long long)lines,
+                  opt.plainSample, plainSpan, plainRate, pp50, pp99);
     std::fputs(js, stdout);
     if (!opt.jsonOut.empty()) writeFile(opt.jsonOut, js);
     ffx_free_pinned(mirror);
     ffx_free(dSlots);
     ffx_free(dMap);
+    ffx_stream_destroy(gst);
+    ffx_event_destroy(gev);
     for (int k = 0; k < S; ++k) {
         ffx_search_plan_destroy(qp[k]);
         ffx_stream_destroy(st[k]);

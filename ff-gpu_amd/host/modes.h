@@ -26,7 +26,7 @@ struct ServeOptions {
     uint32_t limLo = 1u << 18, limHi = 1u << 23;
     double wLo = 6.0, wHi = 18.0;
     double rate = 2000.0;        // requests per second (Poisson)
-    uint32_t maxBatchRequests = 4096;
+    uint64_t batchSums = 65536;   // odd sums per persistent launch (the oldest queued first)
     uint32_t streams = 2;        // batches in flight
     uint32_t plainSample = 0;    // also time this many requests as sequential plain launches
     std::string outDir;

@@ -96,7 +96,7 @@ def test_serve_requests_match_oracle_alone():
     with tempfile.TemporaryDirectory() as d:
         s = summary(ff_sieve("--serve", "--opt", f"count={count}", "--opt", "seed=7", "--opt", "lim_lo=1024",
                              "--opt", "lim_hi=131071", "--opt", "wlo=2", "--opt", "whi=13", "--opt", "rate=3000",
-                             "--opt", "max_batch=8", "--opt", "streams=2", "--opt", f"out_dir={d}"))
+                             "--opt", "batch_sums=512", "--opt", "streams=2", "--opt", f"out_dir={d}"))
         assert s["requests"] == count and s["batches"] >= 2
         reqs = [tuple(map(int, l.split()[:2])) for l in subprocess.run(
             [os.path.join(os.path.dirname(ffxpy.FF_SIEVE), "ff_hostcheck"), "served", "7", str(count), "3000",
@@ -109,10 +109,11 @@ def test_serve_requests_match_oracle_alone():
 
 def test_serve_mix_matches_plain_on_large_sums():
     # requests up to sumLimit 2^23 (capped PRIMEMEM headers included), a
-    # device map bounded by --map-cap; each against a plain run
+    # device map bounded by --map-cap (8 GiB: grown while batches run); each
+    # against a plain run
     count = 24
     with tempfile.TemporaryDirectory() as d:
-        s = summary(ff_sieve("--map-cap", 2 ** 34, "--serve", "--opt", f"count={count}", "--opt", "seed=9",
+        s = summary(ff_sieve("--map-cap", 2 ** 37, "--serve", "--opt", f"count={count}", "--opt", "seed=9",
                              "--opt", "wlo=6", "--opt", "whi=12", "--opt", "rate=500", "--opt", f"out_dir={d}",
                              timeout=600))
         assert s["requests"] == count
@@ -121,7 +122,7 @@ def test_serve_mix_matches_plain_on_large_sums():
              str(1 << 18), str(1 << 23), "6", "12"], capture_output=True, text=True, check=True).stdout.splitlines()]
         for i in random.Random(3).sample(range(count), 6):
             a, b = reqs[i]
-            plain = ff_sieve("--map-cap", 2 ** 34, a, b, timeout=600)
+            plain = ff_sieve("--map-cap", 2 ** 37, a, b, timeout=600)
             assert plain.returncode == 0
             got = normalize(open(os.path.join(d, f"request_{i:05d}.txt")).read())
             assert got == normalize(plain.stdout.decode()), (i, a, b)
