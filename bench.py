@@ -9,7 +9,13 @@ per second of whole steps.  Kernel times come from HIP events on the stream
 the kernels run on.  Parity is checked on every run: the emitted text of the
 last step hashes to the reference's golden.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c3] [--no-cpu]
+Two more legs follow (each its own ff_sieve process, after the plain
+buffers are freed): the batched workload (4096 seeded ranges in one
+graph-captured persistent launch) and the served workload (20000 seeded
+Poisson requests, p50/p99 latency beside sequential plain launches); their
+summaries are attached as "batched" and "served".
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c3] [--no-cpu] [--no-modes]
 """
 import argparse
 import ctypes as C
@@ -93,6 +99,19 @@ def cpu_baseline(sum_limit, timeout_s):
     }
 
 
+def mode_leg(args, timeout_s):
+    """One batched or served run of ff_sieve (its own process, its own
+    device buffers); returns its JSON summary line."""
+    exe = os.path.join(ROOT, "ff-gpu_amd", "bin", "ff_sieve")
+    try:
+        p = subprocess.run([exe] + args, capture_output=True, timeout=timeout_s)
+    except subprocess.TimeoutExpired:
+        return {"error": f"timed out after {timeout_s} s"}
+    if p.returncode != 0:
+        return {"error": p.stderr.decode(errors="replace")[-500:]}
+    return json.loads(p.stdout.decode().strip().splitlines()[-1])
+
+
 def calibrate_copy(nbytes):
     """A plain device-to-device copy of known size (torch's copy kernel),
     three times: FETCH_SIZE and WRITE_SIZE of one dispatch against nbytes
@@ -114,6 +133,7 @@ def main():
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-timeout", type=float, default=150.0)
+    ap.add_argument("--no-modes", action="store_true", help="skip the batched and served legs")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only copy a BYTES device buffer three times: the run that scales the HBM byte counters")
     args = ap.parse_args()
@@ -208,7 +228,19 @@ def main():
                   "frac": round(sieve_gbs / HBM_PEAK_GBS, 4), "traffic": traffic_for("sieve_segments", args.workload),
                   "kernel": "sieve_segments", "algorithmic": f"{map_bytes} map bytes written once"}
 
+    for b in (dmap, slots, out, cnt):
+        b.close()
+    sp.close()
+    qp.close()
     cpu = None if args.no_cpu else cpu_baseline(sum_limit, args.cpu_timeout)
+    batched = served = None
+    if not args.no_modes:
+        # 4096 seeded ranges (starts below 2^22, widths 2^8..2^16) in one
+        # graph-captured persistent launch; the served Poisson stream of
+        # 20000 requests (sumLimit up to 2^23, widths 2^6..2^18, 2000/s)
+        # with 100 of them replayed as sequential plain launches
+        batched = mode_leg(["--batch", "--opt", "reps=3"], 300)
+        served = mode_leg(["--serve", "--opt", "plain_sample=100"], 300)
     res = {
         "metric": METRIC,
         "value": nodd / (ms_per_step * 1e-3),
@@ -235,6 +267,8 @@ def main():
         "solutions": c,
         "parity": "sha256 of emitted stdout == reference golden" if parity else "MISMATCH",
         "cpu_baseline": cpu,
+        "batched": batched,
+        "served": served,
     }
     if cpu and cpu.get("value"):
         res["gpu_over_cpu"] = round(res["value"] / cpu["value"], 1)

@@ -45,7 +45,7 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / 64;
 constexpr int kQueueCap = 192;     // < 64 left after a drain + 128 from one two-round step
-constexpr uint32_t kGrab = 2;      // default sums taken per atomic
+constexpr uint32_t kGrab = 1;      // default sums taken per atomic (the autotuned choice)
 constexpr int kMaxPrimes = 12;     // distinct odd primes of q < 3^31 (q <= s^2/16, s < 2^26)
 constexpr uint32_t kDBits = 27;    // queue entry: d | k << 27
 constexpr int kMaxSegs = 32;

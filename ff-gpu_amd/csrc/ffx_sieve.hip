@@ -10,7 +10,7 @@
 //
 // Kernel design (gfx950, wave64, 160 KiB LDS per CU):
 //  * One workgroup of 1024 threads owns one map segment at a time in LDS
-//    (64 KiB = 2^19 odd values by default, 32 KiB selectable) and walks
+//    (32 KiB = 2^18 odd values by default, 64 KiB selectable) and walks
 //    segments in a grid-stride loop.
 //  * Bits are kept "struck" = 1 in LDS, in natural order (bit b of word w is
 //    odd index 32w+b); the write-out inverts them and reorders each byte to
@@ -32,7 +32,11 @@
 
 namespace {
 
-constexpr uint32_t kDefaultSegBytes = 65536;  // LDS tile: map bytes per segment
+// Defaults measured fastest on MI355X by ff_sieve --autotune (odd sums to
+// 2^20, 4 GiB map): 32 KiB segments, four 1024-thread workgroups per CU (two
+// resident at a time with the 40 KiB of patterns).
+constexpr uint32_t kDefaultSegBytes = 32768;  // LDS tile: map bytes per segment
+constexpr uint32_t kDefaultBlocksPerCU = 4;
 constexpr int kThreads = 1024;
 constexpr int kWaves = kThreads / 64;
 
@@ -271,7 +275,7 @@ static int plan_build(ffx_sieve_plan **out, std::vector<uint32_t> &oddPrimes, ui
     pl->nprimes = (uint32_t)med.size();
     pl->hostPrimes = med;
     pl->segBytes = kDefaultSegBytes;
-    pl->blocksPerCU = 1;
+    pl->blocksPerCU = kDefaultBlocksPerCU;
     hipError_t e = hipGetDevice(&pl->device);
     int cus = 0;
     if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, pl->device);

@@ -241,7 +241,7 @@ int runLeg(const Config &cfg, const std::vector<std::string> &pos)
 }
 
 // key=value options of --batch / --serve (--opt key=value, repeatable)
-bool modeOptions(const Config &cfg, BatchOptions *b, ServeOptions *sv)
+bool modeOptions(const Config &cfg, BatchOptions *b, ServeOptions *sv, AutotuneOptions *at)
 {
     for (const std::string &kv : cfg.modeArgs) {
         const size_t eq = kv.find('=');
@@ -268,6 +268,9 @@ bool modeOptions(const Config &cfg, BatchOptions *b, ServeOptions *sv)
         else if (k == "streams") sv->streams = (uint32_t)u;
         else if (k == "plain_sample") sv->plainSample = (uint32_t)u;
         else if (k == "json") sv->jsonOut = v;
+        else if (k == "sum_limit") at->sumLimit = (uint32_t)u;
+        else if (k == "out") at->out = v;
+        else if (k == "tune_reps") at->reps = (uint32_t)u;
         else {
             std::fprintf(stderr, "[ff_sieve] error: unknown --opt key '%s'\n", k.c_str());
             return false;
@@ -287,11 +290,14 @@ int main(int argc, char **argv)
         std::vector<std::string> pos;
         if (loadEnv(&cfg) || parseArgs(argc, argv, &cfg, &pos) || validateConfig(&cfg)) ret = 1;
         else if (cfg.listDevices) ret = listDevices();
-        else if (cfg.mode == "batch" || cfg.mode == "serve") {
+        else if (cfg.mode == "batch" || cfg.mode == "serve" || cfg.mode == "autotune") {
             BatchOptions b;
             ServeOptions sv;
-            if (!modeOptions(cfg, &b, &sv)) ret = 1;
-            else ret = cfg.mode == "batch" ? runBatch(cfg, b) : runServe(cfg, sv);
+            AutotuneOptions at;
+            if (!modeOptions(cfg, &b, &sv, &at)) ret = 1;
+            else if (cfg.mode == "batch") ret = runBatch(cfg, b);
+            else if (cfg.mode == "serve") ret = runServe(cfg, sv);
+            else ret = runAutotune(cfg, at);
         } else ret = runLeg(cfg, pos);  // no positionals: ff_seg's default leg 5..2627
     }
     phase("total", msSince(t0));

@@ -672,4 +672,172 @@ int runServe(const Config &cfg, const ServeOptions &opt)
     return 0;
 }
 
+// --------------------------------------------------------------------------
+// autotune
+// --------------------------------------------------------------------------
+
+// Times every candidate of the kernels' launch parameters on one workload
+// (odd sums [5, sumLimit] over ff_seg's map for it) and writes the fastest
+// as a tune file for --tune.  Every candidate's output is compared with the
+// default configuration's: the sieve on a smaller map byte for byte, the
+// search slot for slot.
+int runAutotune(const Config &cfg, const AutotuneOptions &opt)
+{
+    DeviceSetup dev;
+    if (setupDevice(cfg, &dev)) return 1;
+    const RangeJob job = makeJob(5, opt.sumLimit, cfg.primeMem);
+    if (job.geom.fatal || !job.any) {
+        std::fprintf(stderr, "[ff_sieve] autotune: empty workload\n");
+        return 1;
+    }
+    const uint64_t M = mapValueFor(job.geom.maxPrimeMapValue, cfg.mapCap);
+    const uint64_t mapBytes = (M >> 4) + 1;
+    const uint64_t nslots = (job.hi - 5) / 2 + 1;
+    const uint32_t reps = std::max(1u, opt.reps);
+    // the sieve's byte-for-byte check runs on a map of at most 256 MiB
+    const uint64_t checkBytes = std::min<uint64_t>(mapBytes, 256ull << 20);
+    if (mapBytes + checkBytes * 2 + nslots * 8 > dev.budget.backing) {
+        std::fprintf(stderr, "[ff_sieve] autotune: workload needs more than the budget; bound it with --map-cap\n");
+        return 1;
+    }
+    int rc;
+    void *stream = ffx_stream_create();
+    void *e0 = ffx_event_create(), *e1 = ffx_event_create();
+    uint8_t *dMap = (uint8_t *)ffx_alloc((mapBytes + 255) & ~255ull);
+    uint8_t *dChk = (uint8_t *)ffx_alloc((checkBytes + 255) & ~255ull);
+    uint32_t *dSlots = (uint32_t *)ffx_alloc(nslots * 4);
+    if (!stream || !e0 || !e1 || !dMap || !dChk || !dSlots) return fail("autotune buffers", FFX_ENOMEM);
+    std::vector<uint8_t> chkRef(checkBytes), chk(checkBytes);
+    std::vector<uint32_t> slotsRef(nslots), slots(nslots);
+
+    auto timeIt = [&](auto &&launch, double *best) -> int {
+        *best = 1e30;
+        for (uint32_t r = 0; r < reps; ++r) {
+            int q;
+            if ((q = ffx_event_record(e0, stream)) || (q = launch()) || (q = ffx_event_record(e1, stream)) ||
+                (q = ffx_event_sync(e1)))
+                return q;
+            double ms = 0;
+            ffx_event_elapsed_ms(e0, e1, &ms);
+            *best = std::min(*best, ms);
+        }
+        return 0;
+    };
+
+    // sieve: LDS tile (segment bytes) x workgroups per CU
+    struct SieveTrial {
+        uint32_t seg, bpc;
+        double ms;
+    };
+    std::vector<SieveTrial> st;
+    ffx_sieve_plan *sp = nullptr;
+    if ((rc = ffx_sieve_plan_create(&sp, M))) return fail("sieve plan", rc);
+    if ((rc = ffx_sieve_map_async(sp, dChk, checkBytes, stream)) || (rc = ffx_stream_sync(stream)) ||
+        (rc = ffx_copy_d2h(chkRef.data(), dChk, checkBytes)))
+        return fail("sieve reference", rc);
+    for (uint32_t seg : {32768u, 65536u})
+        for (uint32_t bpc : {1u, 2u, 3u, 4u}) {
+            if ((rc = ffx_sieve_plan_set(sp, FFX_SIEVE_SEGMENT_BYTES, seg)) ||
+                (rc = ffx_sieve_plan_set(sp, FFX_SIEVE_BLOCKS_PER_CU, bpc)))
+                return fail("sieve set", rc);
+            if ((rc = ffx_memset_async(dChk, 0, checkBytes, stream)) ||
+                (rc = ffx_sieve_map_async(sp, dChk, checkBytes, stream)) || (rc = ffx_stream_sync(stream)) ||
+                (rc = ffx_copy_d2h(chk.data(), dChk, checkBytes)))
+                return fail("sieve candidate", rc);
+            if (chk != chkRef) {
+                std::fprintf(stderr, "[ff_sieve] autotune: sieve segment %u x %u per CU differs from the default\n",
+                             seg, bpc);
+                return 1;
+            }
+            double ms;
+            if ((rc = timeIt([&] { return ffx_sieve_map_async(sp, dMap, mapBytes, stream); }, &ms)))
+                return fail("sieve timing", rc);
+            st.push_back({seg, bpc, ms});
+            std::fprintf(stderr, "[ff_sieve] autotune sieve segment=%u blocks/CU=%u: %.3f ms\n", seg, bpc, ms);
+        }
+    const SieveTrial bs = *std::min_element(st.begin(), st.end(), [](auto &a, auto &b) { return a.ms < b.ms; });
+    // the search reads the map the default configuration wrote
+    if ((rc = ffx_sieve_plan_set(sp, FFX_SIEVE_SEGMENT_BYTES, 65536)) ||
+        (rc = ffx_sieve_map_async(sp, dMap, mapBytes, stream)) || (rc = ffx_stream_sync(stream)))
+        return fail("sieve", rc);
+
+    // search: sums per atomic grab x eager-drain threshold x workgroups per CU
+    struct SearchTrial {
+        uint32_t grab, eager, bpc;
+        double ms;
+    };
+    std::vector<SearchTrial> qt;
+    ffx_search_plan *qp = nullptr;
+    if ((rc = ffx_search_plan_create(&qp, job.hi))) return fail("search plan", rc);
+    uint32_t g0 = 0, e0v = 0, b0 = 0;
+    ffx_search_plan_get(qp, FFX_SEARCH_GRAB, &g0);
+    ffx_search_plan_get(qp, FFX_SEARCH_EAGER_SPLITS, &e0v);
+    ffx_search_plan_get(qp, FFX_SEARCH_BLOCKS_PER_CU, &b0);
+    if ((rc = ffx_search_slots_async(qp, dMap, M, 5, job.hi, dSlots, stream)) || (rc = ffx_stream_sync(stream)) ||
+        (rc = ffx_copy_d2h(slotsRef.data(), dSlots, nslots * 4)))
+        return fail("search reference", rc);
+    double defaultMs = 0;
+    for (uint32_t grab : {1u, 2u, 4u})
+        for (uint32_t eager : {1024u, 4096u, 16384u})
+            for (uint32_t bpc : {4u, 5u, 6u, 8u}) {
+                if ((rc = ffx_search_plan_set(qp, FFX_SEARCH_GRAB, grab)) ||
+                    (rc = ffx_search_plan_set(qp, FFX_SEARCH_EAGER_SPLITS, eager)) ||
+                    (rc = ffx_search_plan_set(qp, FFX_SEARCH_BLOCKS_PER_CU, bpc)))
+                    return fail("search set", rc);
+                double ms;
+                if ((rc = timeIt([&] { return ffx_search_slots_async(qp, dMap, M, 5, job.hi, dSlots, stream); }, &ms)))
+                    return fail("search timing", rc);
+                if ((rc = ffx_copy_d2h(slots.data(), dSlots, nslots * 4))) return fail("slot copy", rc);
+                if (slots != slotsRef) {
+                    std::fprintf(stderr, "[ff_sieve] autotune: search grab=%u eager=%u blocks/CU=%u differs\n", grab,
+                                 eager, bpc);
+                    return 1;
+                }
+                if (grab == g0 && eager == e0v && bpc == b0) defaultMs = ms;
+                qt.push_back({grab, eager, bpc, ms});
+                std::fprintf(stderr, "[ff_sieve] autotune search grab=%u eager=%u blocks/CU=%u: %.3f ms\n", grab,
+                             eager, bpc, ms);
+            }
+    const SearchTrial bq = *std::min_element(qt.begin(), qt.end(), [](auto &a, auto &b) { return a.ms < b.ms; });
+
+    // the chosen values first (the tune-file reader takes the first match)
+    std::string js;
+    char buf[1024];
+    std::snprintf(buf, sizeof buf,
+                  "{\"search_grab\": %u, \"search_eager_splits\": %u, \"search_blocks_per_cu\": %u, "
+                  "\"sieve_segment_bytes\": %u, \"sieve_blocks_per_cu\": %u,\n \"workload\": \"odd sums [5, %u], map "
+                  "M=%llu\", \"device\": \"%s %s, %d CUs\",\n \"search_ms\": %.3f, \"search_default_ms\": %.3f, "
+                  "\"sieve_ms\": %.3f,\n",
+                  bq.grab, bq.eager, bq.bpc, bs.seg, bs.bpc, job.hi, (unsigned long long)M, dev.info.name[0] ? dev.info.name : "AMD",
+                  dev.info.arch, dev.info.multiProcessorCount, bq.ms, defaultMs, bs.ms);
+    js += buf;
+    js += " \"trials\": {\"sieve\": [";
+    for (size_t i = 0; i < st.size(); ++i) {
+        std::snprintf(buf, sizeof buf, "%s[%u, %u, %.3f]", i ? ", " : "", st[i].seg, st[i].bpc, st[i].ms);
+        js += buf;
+    }
+    js += "],\n  \"search\": [";
+    for (size_t i = 0; i < qt.size(); ++i) {
+        std::snprintf(buf, sizeof buf, "%s[%u, %u, %u, %.3f]", i ? ", " : "", qt[i].grab, qt[i].eager, qt[i].bpc,
+                      qt[i].ms);
+        js += buf;
+    }
+    js += "],\n  \"columns\": {\"sieve\": [\"segment bytes\", \"blocks per CU\", \"ms\"], \"search\": [\"grab\", "
+          "\"eager splits\", \"blocks per CU\", \"ms\"]}}}\n";
+    if (!writeFile(opt.out, js)) {
+        std::fprintf(stderr, "[ff_sieve] error: cannot write %s\n", opt.out.c_str());
+        return 1;
+    }
+    std::fputs(js.c_str(), stdout);
+    ffx_search_plan_destroy(qp);
+    ffx_sieve_plan_destroy(sp);
+    ffx_free(dSlots);
+    ffx_free(dChk);
+    ffx_free(dMap);
+    ffx_event_destroy(e0);
+    ffx_event_destroy(e1);
+    ffx_stream_destroy(stream);
+    return 0;
+}
+
 }  // namespace ffg

@@ -33,7 +33,14 @@ struct ServeOptions {
     std::string jsonOut;         // summary line
 };
 
+struct AutotuneOptions {
+    uint32_t sumLimit = 1048575;  // workload: odd sums [5, sumLimit] on ff_seg's map for it
+    uint32_t reps = 2;            // timed launches per candidate (the minimum counts)
+    std::string out = "ffx_tune.json";
+};
+
 int runBatch(const Config &cfg, const BatchOptions &opt);
+int runAutotune(const Config &cfg, const AutotuneOptions &opt);
 int runServe(const Config &cfg, const ServeOptions &opt);
 
 }  // namespace ffg

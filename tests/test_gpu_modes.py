@@ -14,8 +14,10 @@ import tempfile
 
 import pytest
 
+import hashlib
+
 import ffxpy
-from conftest import normalize, run_ffref
+from conftest import golden_sha, normalize, run_ffref
 
 pytestmark = pytest.mark.gpu
 
@@ -126,3 +128,18 @@ def test_serve_mix_matches_plain_on_large_sums():
             assert plain.returncode == 0
             got = normalize(open(os.path.join(d, f"request_{i:05d}.txt")).read())
             assert got == normalize(plain.stdout.decode()), (i, a, b)
+
+
+def test_autotune_writes_a_tune_file_that_keeps_stdout():
+    # every candidate is checked against the default configuration inside
+    # the autotuner; the file it writes drives a plain run to the golden
+    with tempfile.TemporaryDirectory() as d:
+        f = os.path.join(d, "tune.json")
+        p = ff_sieve("--autotune", "--opt", "sum_limit=65535", "--opt", "tune_reps=1", "--opt", f"out={f}")
+        assert p.returncode == 0, p.stderr.decode()[-2000:]
+        t = json.load(open(f))
+        assert t["sieve_segment_bytes"] in (32768, 65536) and 1 <= t["search_blocks_per_cu"] <= 16
+        assert len(t["trials"]["search"]) == 36 and len(t["trials"]["sieve"]) == 8
+        p = ff_sieve("--tune", f, 5, 65535)
+        assert p.returncode == 0
+        assert hashlib.sha256(normalize(p.stdout.decode()).encode()).hexdigest() == golden_sha(65536)
