@@ -24,8 +24,10 @@
 // from one atomic counter, largest sums first.  Splits are visited k
 // descending as the reference does (it stops far earlier on non-solutions
 // than k ascending), flattened over all k so every lane is busy, two rounds
-// of 64 in flight per wave.  A probe value with a prime factor in 5..31 is
-// rejected by residue tables in LDS before it costs a random map read.
+// of 64 in flight per wave.  A probe value with a prime factor in 3..37 is
+// rejected by residue tables in LDS before it costs a random map read; the
+// residues of both probe values of a split come from those of d, o and 2^k,
+// and both are filtered before the split's first read.
 // Splits that pass both PTP probes go to a per-wave LDS queue; while a sum is
 // young every queued candidate is resolved at once (early exit), later they
 // are resolved 64 at a time, one per lane, so the rare factor-pair
@@ -55,12 +57,15 @@ constexpr int kMaxSegs = 32;
 constexpr uint32_t kEagerSplits = 4096;
 constexpr uint32_t kBlocksPerCU = 8;  // default persistent grid: CUs x this
 
-// Residue filter: a probe value larger than 31 with a prime factor in 3..31
+// Residue filter: a probe value larger than 37 with a prime factor in 3..37
 // is composite without touching the map.  One bit table per
-// modulus, bit r set when gcd(r, m) == 1.
+// modulus, bit r set when gcd(r, m) == 1.  Each modulus is below 2^16, so
+// residues of the two probe values 2^k + d*o - 2 and d + o*2^k - 2 come from
+// the 32-bit residues of d, o and 2^k with 32-bit products.
 constexpr uint32_t kM1 = 3 * 5 * 7 * 11 * 13;  // 15015
 constexpr uint32_t kM2 = 17 * 19 * 23;     // 7429
-constexpr uint32_t kM3 = 29 * 31;          // 899
+constexpr uint32_t kM3 = 29 * 31 * 37;     // 33263
+constexpr uint32_t kLargestResiduePrime = 37;
 constexpr uint32_t kT1 = (kM1 + 31) / 32, kT2 = (kM2 + 31) / 32, kT3 = (kM3 + 31) / 32;
 constexpr uint32_t kTabWords = kT1 + kT2 + kT3;
 
@@ -200,21 +205,29 @@ __device__ __forceinline__ uint64_t isqrt53(uint64_t v)
     return r;
 }
 
-template <uint32_t m>
-__device__ __forceinline__ uint32_t mod64(uint64_t x)
-{
-    constexpr uint32_t c = (uint32_t)((1ull << 32) % m);
-    const uint32_t t = ((uint32_t)(x >> 32) % m) * c + (uint32_t)x % m;  // < m^2 + m < 2^32
-    return t % m;
-}
-
 __device__ __forceinline__ bool tab_bit(const uint32_t *t, uint32_t r) { return (t[r >> 5] >> (r & 31)) & 1; }
 
-// False when odd x > 31 has a prime factor in 3..31 (x is then composite).
-__device__ __forceinline__ bool residue_ok(const uint32_t *tab, uint64_t x)
+// Coprimality to m of X = 2^k + d*o - 2 and Y = d + o*2^k - 2 (pk = 2^k),
+// from residues: m^2 + 2m < 2^32 keeps every intermediate in 32 bits.
+template <uint32_t m>
+__device__ __forceinline__ void residue_pair(const uint32_t *t, uint32_t d, uint32_t o, uint32_t pk, bool &rx, bool &ry)
 {
-    if (x <= 31) return true;
-    return tab_bit(tab, mod64<kM1>(x)) && tab_bit(tab + kT1, mod64<kM2>(x)) && tab_bit(tab + kT1 + kT2, mod64<kM3>(x));
+    static_assert((uint64_t)m * m + 2 * m < (1ull << 32), "residue products must fit 32 bits");
+    const uint32_t dm = d % m, om = o % m, pm = pk % m;
+    rx = rx & tab_bit(t, (dm * om + pm + m - 2) % m);
+    ry = ry & tab_bit(t, (dm + om * pm + m - 2) % m);
+}
+
+// rx (ry) false when X (Y) has a prime factor in 3..37; X, Y > 37 are then
+// composite.
+__device__ __forceinline__ void probe_residues(const uint32_t *tab, uint32_t d, uint32_t o, uint32_t k, bool &rx,
+                                               bool &ry)
+{
+    rx = ry = true;
+    const uint32_t pk = 1u << k;
+    residue_pair<kM1>(tab, d, o, pk, rx, ry);
+    residue_pair<kM2>(tab + kT1, d, o, pk, rx, ry);
+    residue_pair<kM3>(tab + kT1 + kT2, d, o, pk, rx, ry);
 }
 
 // ---------------------------------------------------------------- AB1
@@ -457,13 +470,20 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
         for (int u = 0; u < 2; ++u)
             if (needO[u]) st[u] = sp[u].o <= a.M ? byte_bit(ob[u], sp[u].o) : mr_is_prime(sp[u].o);
 
-        // stage 2: PTP(2^k + q), q = d*o
-        uint64_t x[2];
+        // residue filter of both probe values before any map read
+        uint64_t x[2], y[2];
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
             x[u] = (1ull << sp[u].k) + (uint64_t)sp[u].d * sp[u].o - 2;
-            st[u] = st[u] && residue_ok(tab, x[u]);
+            y[u] = (uint64_t)sp[u].d + ((uint64_t)sp[u].o << sp[u].k) - 2;
+            bool rx, ry;
+            probe_residues(tab, sp[u].d, sp[u].o, sp[u].k, rx, ry);
+            rx = rx || x[u] <= kLargestResiduePrime;
+            ry = ry || y[u] <= kLargestResiduePrime || sp[u].d == sp[u].o;
+            st[u] = st[u] && rx && ry;
         }
+
+        // stage 2: PTP(2^k + q), q = d*o
 #pragma unroll
         for (int u = 0; u < 2; ++u) ob[u] = (st[u] && x[u] <= a.M) ? map_byte(a.map, x[u]) : 0u;
 #pragma unroll
@@ -473,16 +493,12 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
         // stage 3: PTP(d + o*2^k) unless d == o
         bool need2[2];
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            need2[u] = st[u] && sp[u].d != sp[u].o;
-            x[u] = (uint64_t)sp[u].d + ((uint64_t)sp[u].o << sp[u].k) - 2;
-            if (need2[u]) st[u] = residue_ok(tab, x[u]);
-        }
+        for (int u = 0; u < 2; ++u) need2[u] = st[u] && sp[u].d != sp[u].o;
 #pragma unroll
-        for (int u = 0; u < 2; ++u) ob[u] = (need2[u] && st[u] && x[u] <= a.M) ? map_byte(a.map, x[u]) : 0u;
+        for (int u = 0; u < 2; ++u) ob[u] = (need2[u] && y[u] <= a.M) ? map_byte(a.map, y[u]) : 0u;
 #pragma unroll
         for (int u = 0; u < 2; ++u)
-            if (need2[u] && st[u]) st[u] = x[u] <= a.M ? byte_bit(ob[u], x[u]) : mr_is_prime(x[u]);
+            if (need2[u]) st[u] = y[u] <= a.M ? byte_bit(ob[u], y[u]) : mr_is_prime(y[u]);
 
         // survivors wait in the queue for AllButOne (:638)
 #pragma unroll
