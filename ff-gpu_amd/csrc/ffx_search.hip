@@ -24,8 +24,8 @@
 // from one atomic counter, largest sums first.  Splits are visited k
 // descending as the reference does (it stops far earlier on non-solutions
 // than k ascending), flattened over all k so every lane is busy, two rounds
-// of 64 in flight per wave.  A probe value with a prime factor in 3..37 is
-// rejected by residue tables in LDS before it costs a random map read; the
+// of 64 in flight per wave.  A probe value with a prime factor in 3..13 is
+// rejected by a residue table in LDS before it costs a random map read; the
 // residues of both probe values of a split come from those of d, o and 2^k,
 // and both are filtered before the split's first read.
 // Splits that pass both PTP probes go to a per-wave LDS queue; while a sum is
@@ -57,17 +57,25 @@ constexpr int kMaxSegs = 32;
 constexpr uint32_t kEagerSplits = 4096;
 constexpr uint32_t kBlocksPerCU = 8;  // default persistent grid: CUs x this
 
-// Residue filter: a probe value larger than 37 with a prime factor in 3..37
-// is composite without touching the map.  One bit table per
-// modulus, bit r set when gcd(r, m) == 1.  Each modulus is below 2^16, so
+// Residue filter: a probe value larger than the largest prime of the moduli
+// in use that shares a factor with one of them is composite without touching
+// the map.  One bit table per modulus, bit r set when gcd(r, m) == 1.  Each
+// modulus is below 2^16, so
 // residues of the two probe values 2^k + d*o - 2 and d + o*2^k - 2 come from
 // the 32-bit residues of d, o and 2^k with 32-bit products.
-constexpr uint32_t kM1 = 3 * 5 * 7 * 11 * 13;  // 15015
-constexpr uint32_t kM2 = 17 * 19 * 23;     // 7429
-constexpr uint32_t kM3 = 29 * 31 * 37;     // 33263
-constexpr uint32_t kLargestResiduePrime = 37;
-constexpr uint32_t kT1 = (kM1 + 31) / 32, kT2 = (kM2 + 31) / 32, kT3 = (kM3 + 31) / 32;
-constexpr uint32_t kTabWords = kT1 + kT2 + kT3;
+constexpr uint32_t kMods[5] = {3 * 5 * 7 * 11 * 13, 17 * 19 * 23, 29 * 31 * 37, 41 * 43, 47 * 53};
+constexpr uint32_t kModTopPrime[5] = {13, 23, 37, 43, 53};
+// One modulus measured fastest on MI355X: each further modulus costs more
+// integer issue than the map reads it saves ([5, 2^20): 40.5 / 41.7 / 43.7 /
+// 45.7 / 49.4 ms with 1..5 moduli).
+#ifndef FFX_RESIDUE_MODULI
+#define FFX_RESIDUE_MODULI 1
+#endif
+constexpr int kNMods = FFX_RESIDUE_MODULI;
+static_assert(kNMods >= 1 && kNMods <= 5, "1..5 residue moduli");
+constexpr uint32_t tab_offset(int i) { return i == 0 ? 0 : tab_offset(i - 1) + (kMods[i - 1] + 31) / 32; }
+constexpr uint32_t kTabWords = tab_offset(kNMods);
+constexpr uint32_t kLargestResiduePrime = kModTopPrime[kNMods - 1];
 
 struct SearchArgs {
     const uint8_t *map;
@@ -218,16 +226,23 @@ __device__ __forceinline__ void residue_pair(const uint32_t *t, uint32_t d, uint
     ry = ry & tab_bit(t, (dm + om * pm + m - 2) % m);
 }
 
-// rx (ry) false when X (Y) has a prime factor in 3..37; X, Y > 37 are then
-// composite.
+template <int I>
+__device__ __forceinline__ void residue_all(const uint32_t *tab, uint32_t d, uint32_t o, uint32_t pk, bool &rx, bool &ry)
+{
+    if constexpr (I < kNMods) {
+        residue_pair<kMods[I]>(tab + tab_offset(I), d, o, pk, rx, ry);
+        residue_all<I + 1>(tab, d, o, pk, rx, ry);
+    }
+}
+
+// rx (ry) false when X (Y) has a prime factor in 3..kLargestResiduePrime;
+// X, Y above that prime are then composite.
 __device__ __forceinline__ void probe_residues(const uint32_t *tab, uint32_t d, uint32_t o, uint32_t k, bool &rx,
                                                bool &ry)
 {
     rx = ry = true;
     const uint32_t pk = 1u << k;
-    residue_pair<kM1>(tab, d, o, pk, rx, ry);
-    residue_pair<kM2>(tab + kT1, d, o, pk, rx, ry);
-    residue_pair<kM3>(tab + kT1 + kT2, d, o, pk, rx, ry);
+    residue_all<0>(tab, d, o, pk, rx, ry);
 }
 
 // ---------------------------------------------------------------- AB1
@@ -739,16 +754,15 @@ int ffx_search_plan_create(ffx_search_plan **out, uint32_t sumLimitMax)
     if (e == hipSuccess) e = hipMalloc(&pl->dCounter, 64);
     // coprimality bit tables of the residue filter
     std::vector<uint32_t> tab(kTabWords, 0);
-    const uint32_t mods[3] = {kM1, kM2, kM3}, offs[3] = {0, kT1, kT1 + kT2};
-    for (int t = 0; t < 3; ++t)
-        for (uint32_t r = 0; r < mods[t]; ++r) {
-            uint32_t g = r, m = mods[t];
+    for (int t = 0; t < kNMods; ++t)
+        for (uint32_t r = 0; r < kMods[t]; ++r) {
+            uint32_t g = r, m = kMods[t];
             while (m) {
                 const uint32_t tmp = g % m;
                 g = m;
                 m = tmp;
             }
-            if (g == 1) tab[offs[t] + r / 32] |= 1u << (r % 32);
+            if (g == 1) tab[tab_offset(t) + r / 32] |= 1u << (r % 32);
         }
     if (e == hipSuccess) e = hipMalloc(&pl->dCoprime, kTabWords * 4);
     if (e == hipSuccess) e = hipMemcpy(pl->dCoprime, tab.data(), kTabWords * 4, hipMemcpyHostToDevice);
