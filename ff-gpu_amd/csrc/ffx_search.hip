@@ -427,25 +427,41 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
 
     // Power2Odd and Power2Even splits, k descending as the reference visits
     // them (:570-572, :611-613), both families interleaved by k, flattened
-    // into one index space: segment j is k = kTop - j, d = d0, d0+2, ...
+    // into one index space: segment j is k = kTop - j with its d sequence.
+    // Power2Odd (k odd) needs 3 !| d and 3 !| o = s - d*2^k, i.e. d mod 3
+    // outside {0, 2s mod 3} (2^k = 2 mod 3): the segment lists only those d
+    // -- d = 1 mod 6 from 7 when s = 1 mod 3, d = 5 mod 6 when s = 2 mod 3,
+    // d = 5, 7, 11, 13, ... when 3 | s -- so no lane visits a split its
+    // family rejects.  Power2Even (k even) lists every odd d >= 3.
     const uint32_t t3 = (s - 3) / 3;
     const int kTop = t3 >= 4 ? 31 - __clz(t3) : 0;  // largest k with 3*2^k <= s-3
     const int nseg = kTop >= 2 ? kTop - 1 : 0;
+    const uint32_t s3 = s % 3;
     uint32_t total = 0;
     {
-        uint32_t n = 0, kj = 0, d0 = 0;
+        uint32_t n = 0, kj = 0, d0 = 0, mode = 0;
         if (lane < nseg) {
             kj = (uint32_t)(kTop - lane);
-            d0 = (kj & 1) ? 5 : 3;
             const uint32_t D = (s - 3) >> kj;  // d*2^k < s-2  <=>  d <= (s-3)>>k
-            n = D >= d0 ? (D - d0) / 2 + 1 : 0;
+            if (!(kj & 1)) {
+                d0 = 3;
+                n = D >= d0 ? (D - d0) / 2 + 1 : 0;
+            } else if (s3 != 0) {
+                mode = 1;
+                d0 = s3 == 1 ? 7 : 5;
+                n = D >= d0 ? (D - d0) / 6 + 1 : 0;
+            } else {
+                mode = 2;
+                d0 = 5;
+                n = D >= 5 ? 2 * ((D - 5) / 6) + ((D - 5) % 6 >= 2 ? 2 : 1) : 0;
+            }
         }
         uint32_t incl = n;
         for (int off = 1; off < 64; off <<= 1) {
             const uint32_t v = __shfl_up(incl, off);
             if (lane >= off) incl += v;
         }
-        if (lane < nseg) segs[lane] = make_uint2(incl - n, kj | (d0 << 8));
+        if (lane < nseg) segs[lane] = make_uint2(incl - n, kj | (mode << 5) | (d0 << 8));
         total = nseg ? __shfl(incl, nseg - 1) : 0;
     }
     wave_sync_lds();
@@ -463,8 +479,9 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
                 while (j + 1 < nseg && g >= segs[j + 1].x) ++j;
             segOf[u] = j;
             const uint2 e = segs[j];
-            sp[u].k = e.y & 255;
-            sp[u].d = (e.y >> 8) + 2 * (g - e.x);
+            sp[u].k = e.y & 31;
+            const uint32_t mode = (e.y >> 5) & 3, i = g - e.x;
+            sp[u].d = (e.y >> 8) + (mode == 0 ? 2 * i : mode == 1 ? 6 * i : 6 * (i >> 1) + 2 * (i & 1));
             sp[u].o = sp[u].live ? s - (sp[u].d << sp[u].k) : 1;
         }
         hint = __shfl(segOf[1], 63);
