@@ -63,11 +63,11 @@ constexpr uint32_t kBlocksPerCU = 8;  // default persistent grid: CUs x this
 // modulus is below 2^16, so
 // residues of the two probe values 2^k + d*o - 2 and d + o*2^k - 2 come from
 // the 32-bit residues of d, o and 2^k with 32-bit products.
-constexpr uint32_t kMods[5] = {3 * 5 * 7 * 11 * 13, 17 * 19 * 23, 29 * 31 * 37, 41 * 43, 47 * 53};
-constexpr uint32_t kModTopPrime[5] = {13, 23, 37, 43, 53};
+constexpr uint32_t kMods[5] = {3 * 5 * 7 * 11 * 13, 17 * 19 * 23, 29 * 31, 37 * 41, 43 * 47};
+constexpr uint32_t kModTopPrime[5] = {13, 23, 31, 41, 47};
 // One modulus measured fastest on MI355X: each further modulus costs more
-// integer issue than the map reads it saves ([5, 2^20): 40.5 / 41.7 / 43.7 /
-// 45.7 / 49.4 ms with 1..5 moduli).
+// integer issue than the map reads it saves ([5, 2^20) search: 40.5 ms with
+// one modulus, 41.7 with two, 43.7 with a third).
 #ifndef FFX_RESIDUE_MODULI
 #define FFX_RESIDUE_MODULI 1
 #endif
@@ -215,15 +215,37 @@ __device__ __forceinline__ uint64_t isqrt53(uint64_t v)
 
 __device__ __forceinline__ bool tab_bit(const uint32_t *t, uint32_t r) { return (t[r >> 5] >> (r & 31)) & 1; }
 
+// n mod m for n < 2^28 with one high multiply: c = ceil(2^(32+sh) / m) and
+// e = c*m - 2^(32+sh) < 2^(sh+4) make floor(n*c / 2^(32+sh)) = floor(n/m)
+// for every n < 2^28 (n*e < 2^(32+sh)).  The general 32-bit division by a
+// constant needs three more instructions.
+constexpr int floor_log2(uint32_t m) { return m <= 1 ? 0 : 1 + floor_log2(m >> 1); }
+
+template <uint32_t m>
+struct Mod28 {
+    static constexpr int sh = floor_log2(m);  // c < 2^32 for m not a power of two
+    static constexpr uint64_t c = ((1ull << (32 + sh)) + m - 1) / m;
+    static_assert(c < (1ull << 32), "multiplier must fit 32 bits");
+    static_assert(c * m - (1ull << (32 + sh)) < (1ull << (sh + 4)), "quotient must be exact below 2^28");
+};
+
+template <uint32_t m>
+__device__ __forceinline__ uint32_t mod28(uint32_t n)
+{
+    const uint32_t q = __umulhi(n, (uint32_t)Mod28<m>::c) >> Mod28<m>::sh;
+    return n - q * m;
+}
+
 // Coprimality to m of X = 2^k + d*o - 2 and Y = d + o*2^k - 2 (pk = 2^k),
-// from residues: m^2 + 2m < 2^32 keeps every intermediate in 32 bits.
+// from residues; d, o, pk < 2^26 and m^2 + 2m < 2^28 keep every reduced
+// value below 2^28.
 template <uint32_t m>
 __device__ __forceinline__ void residue_pair(const uint32_t *t, uint32_t d, uint32_t o, uint32_t pk, bool &rx, bool &ry)
 {
-    static_assert((uint64_t)m * m + 2 * m < (1ull << 32), "residue products must fit 32 bits");
-    const uint32_t dm = d % m, om = o % m, pm = pk % m;
-    rx = rx & tab_bit(t, (dm * om + pm + m - 2) % m);
-    ry = ry & tab_bit(t, (dm + om * pm + m - 2) % m);
+    static_assert((uint64_t)m * m + 2 * m < (1ull << 28), "residue products must stay below 2^28");
+    const uint32_t dm = mod28<m>(d), om = mod28<m>(o), pm = mod28<m>(pk);
+    rx = rx & tab_bit(t, mod28<m>(dm * om + pm + m - 2));
+    ry = ry & tab_bit(t, mod28<m>(dm + om * pm + m - 2));
 }
 
 template <int I>
