@@ -46,7 +46,9 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / 64;
-constexpr int kQueueCap = 192;     // < 64 left after a drain + 128 from one two-round step
+constexpr int kQueueCap = 192;     // < 64 left after a drain + 128 from one probe batch
+constexpr int kProbeCap = 256;     // < 128 left after the batches + 128 from one round
+constexpr uint32_t kNeedO = 1u << 31;  // probe entry: o must be prime too (Power2Even, 3 !| d*o)
 constexpr uint32_t kGrab = 1;      // default sums taken per atomic (the autotuned choice)
 constexpr int kMaxPrimes = 12;     // distinct odd primes of q < 3^31 (q <= s^2/16, s < 2^26)
 constexpr uint32_t kDBits = 27;    // queue entry: d | k << 27
@@ -240,31 +242,34 @@ __device__ __forceinline__ uint32_t mod28(uint32_t n)
 // from residues; d, o, pk < 2^26 and m^2 + 2m < 2^28 keep every reduced
 // value below 2^28.
 template <uint32_t m>
-__device__ __forceinline__ void residue_pair(const uint32_t *t, uint32_t d, uint32_t o, uint32_t pk, bool &rx, bool &ry)
+__device__ __forceinline__ void residue_pair(const uint32_t *t, uint32_t d, uint32_t o, uint32_t pk, bool &rx, bool &ry,
+                                             bool &ro)
 {
     static_assert((uint64_t)m * m + 2 * m < (1ull << 28), "residue products must stay below 2^28");
     const uint32_t dm = mod28<m>(d), om = mod28<m>(o), pm = mod28<m>(pk);
     rx = rx & tab_bit(t, mod28<m>(dm * om + pm + m - 2));
     ry = ry & tab_bit(t, mod28<m>(dm + om * pm + m - 2));
+    ro = ro & tab_bit(t, om);
 }
 
 template <int I>
-__device__ __forceinline__ void residue_all(const uint32_t *tab, uint32_t d, uint32_t o, uint32_t pk, bool &rx, bool &ry)
+__device__ __forceinline__ void residue_all(const uint32_t *tab, uint32_t d, uint32_t o, uint32_t pk, bool &rx, bool &ry,
+                                            bool &ro)
 {
     if constexpr (I < kNMods) {
-        residue_pair<kMods[I]>(tab + tab_offset(I), d, o, pk, rx, ry);
-        residue_all<I + 1>(tab, d, o, pk, rx, ry);
+        residue_pair<kMods[I]>(tab + tab_offset(I), d, o, pk, rx, ry, ro);
+        residue_all<I + 1>(tab, d, o, pk, rx, ry, ro);
     }
 }
 
-// rx (ry) false when X (Y) has a prime factor in 3..kLargestResiduePrime;
-// X, Y above that prime are then composite.
+// rx (ry, ro) false when X (Y, o) has a prime factor in
+// 3..kLargestResiduePrime; values above that prime are then composite.
 __device__ __forceinline__ void probe_residues(const uint32_t *tab, uint32_t d, uint32_t o, uint32_t k, bool &rx,
-                                               bool &ry)
+                                               bool &ry, bool &ro)
 {
-    rx = ry = true;
+    rx = ry = ro = true;
     const uint32_t pk = 1u << k;
-    residue_all<0>(tab, d, o, pk, rx, ry);
+    residue_all<0>(tab, d, o, pk, rx, ry, ro);
 }
 
 // ---------------------------------------------------------------- AB1
@@ -410,17 +415,59 @@ __device__ __forceinline__ void drain(const SearchArgs &a, const uint2 *queue, i
     }
 }
 
-// One split (k, d) of sum s through the PTP probes of DoesPeterKnow
-// (:632-639) and the family filter of Power2Odd/Power2Even (:578, :619).
-// Two splits per lane are carried through each stage together so their
-// random map reads are in flight at the same time.
-struct Split {
-    uint32_t k, d, o;
-    bool live;
-};
+// Map probes of queued splits (entries [from, from+nb) of the wave's probe
+// queue, two per lane so their random map reads are in flight together):
+// the prime-o condition of Power2Even (:619), PTP(2^k + q) and PTP(d + o*2^k)
+// of DoesPeterKnow (:632-639).  Survivors go to the AllButOne queue.
+__device__ __forceinline__ void probe_batch(const SearchArgs &a, const uint2 *pq, int from, int nb, int lane,
+                                            uint64_t lt, uint2 *queue, int &qn)
+{
+    uint32_t d[2], k[2], o[2], ob[2], oo[2];
+    bool st[2], needO[2];
+    uint64_t x[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int idx = 64 * u + lane;
+        st[u] = idx < nb;
+        const uint2 e = st[u] ? pq[from + idx] : make_uint2(3u | (2u << kDBits), 1u);
+        d[u] = e.x & ((1u << kDBits) - 1);
+        k[u] = e.x >> kDBits;
+        o[u] = e.y & ~kNeedO;
+        needO[u] = st[u] && (e.y & kNeedO);
+        x[u] = (1ull << k[u]) + (uint64_t)d[u] * o[u] - 2;
+    }
+    // o and 2^k + q - 2 read together
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        oo[u] = (needO[u] && o[u] <= a.M) ? map_byte(a.map, o[u]) : 0u;
+        ob[u] = (st[u] && x[u] <= a.M) ? map_byte(a.map, x[u]) : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        if (needO[u]) st[u] = o[u] <= a.M ? byte_bit(oo[u], o[u]) : mr_is_prime(o[u]);
+        if (st[u]) st[u] = x[u] <= a.M ? byte_bit(ob[u], x[u]) : mr_is_prime(x[u]);
+    }
+    // d + o*2^k - 2 unless d == o
+    bool need2[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        need2[u] = st[u] && d[u] != o[u];
+        x[u] = (uint64_t)d[u] + ((uint64_t)o[u] << k[u]) - 2;
+        ob[u] = (need2[u] && x[u] <= a.M) ? map_byte(a.map, x[u]) : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+        if (need2[u]) st[u] = x[u] <= a.M ? byte_bit(ob[u], x[u]) : mr_is_prime(x[u]);
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const uint64_t cm = __ballot(st[u]);
+        if (st[u]) queue[qn + __popcll(cm & lt)] = make_uint2(d[u] | (k[u] << kDBits), o[u]);
+        qn += __popcll(cm);
+    }
+}
 
 __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const uint32_t *tab, uint32_t s, int lane,
-                                                   uint2 *queue, uint2 *segs, uint32_t *pr)
+                                                   uint2 *queue, uint2 *pq, uint2 *segs, uint32_t *pr)
 {
     // statement 1 (RunIt :806-807)
     if (is_prime_odd(a, s - 2)) return 0;
@@ -488,87 +535,59 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
     }
     wave_sync_lds();
 
-    int hint = 0;
+    // Each round: 128 splits (two per lane) through the filters that need no
+    // map read -- the family test of Power2Odd/Power2Even (:578, :619) and
+    // the residue filter of both probe values (and of o where Power2Even
+    // needs it prime); the few survivors are compacted into the probe queue
+    // and probed 128 at a time, so the map reads run on full waves.  While
+    // the sum is young, queues are emptied every round (early exit at two
+    // valid splits).
+    int hint = 0, pn = 0;
     for (uint32_t base = 0; base < total; base += 128) {
-        Split sp[2];
         int segOf[2];
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
             const uint32_t g = base + 64u * u + lane;
-            sp[u].live = g < total;
+            const bool live = g < total;
             int j = hint;
-            if (sp[u].live)
+            if (live)
                 while (j + 1 < nseg && g >= segs[j + 1].x) ++j;
             segOf[u] = j;
             const uint2 e = segs[j];
-            sp[u].k = e.y & 31;
-            const uint32_t mode = (e.y >> 5) & 3, i = g - e.x;
-            sp[u].d = (e.y >> 8) + (mode == 0 ? 2 * i : mode == 1 ? 6 * i : 6 * (i >> 1) + 2 * (i & 1));
-            sp[u].o = sp[u].live ? s - (sp[u].d << sp[u].k) : 1;
+            const uint32_t k = e.y & 31, mode = (e.y >> 5) & 3, i = g - e.x;
+            const uint32_t d = (e.y >> 8) + (mode == 0 ? 2 * i : mode == 1 ? 6 * i : 6 * (i >> 1) + 2 * (i & 1));
+            const uint32_t o = live ? s - (d << k) : 1;
+            const bool m3 = (d % 3 == 0) || (o % 3 == 0);
+            const bool odd = k & 1;
+            const bool needO = !odd && !m3;  // Power2Even without 3 | d*o: o must be prime
+            bool rx, ry, ro;
+            probe_residues(tab, d, o, k, rx, ry, ro);
+            const uint64_t x = (1ull << k) + (uint64_t)d * o - 2;
+            const uint64_t y = (uint64_t)d + ((uint64_t)o << k) - 2;
+            rx = rx || x <= kLargestResiduePrime;
+            ry = ry || y <= kLargestResiduePrime || d == o;
+            ro = ro || o <= kLargestResiduePrime;
+            const bool keep = live && (odd ? !m3 : true) && rx && ry && (!needO || ro);
+            const uint64_t km = __ballot(keep);
+            if (keep) pq[pn + __popcll(km & lt)] = make_uint2(d | (k << kDBits), o | (needO ? kNeedO : 0u));
+            pn += __popcll(km);
         }
         hint = __shfl(segOf[1], 63);
-
-        // stage 1: 3 | d*o decides the family; Power2Even also takes prime o
-        bool st[2], needO[2];
-#pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            const bool m3 = (sp[u].d % 3 == 0) || (sp[u].o % 3 == 0);
-            const bool odd = sp[u].k & 1;
-            needO[u] = sp[u].live && !odd && !m3;
-            st[u] = sp[u].live && (odd ? !m3 : true);
-        }
-        uint32_t ob[2];
-#pragma unroll
-        for (int u = 0; u < 2; ++u) ob[u] = (needO[u] && sp[u].o <= a.M) ? map_byte(a.map, sp[u].o) : 0u;
-#pragma unroll
-        for (int u = 0; u < 2; ++u)
-            if (needO[u]) st[u] = sp[u].o <= a.M ? byte_bit(ob[u], sp[u].o) : mr_is_prime(sp[u].o);
-
-        // residue filter of both probe values before any map read
-        uint64_t x[2], y[2];
-#pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            x[u] = (1ull << sp[u].k) + (uint64_t)sp[u].d * sp[u].o - 2;
-            y[u] = (uint64_t)sp[u].d + ((uint64_t)sp[u].o << sp[u].k) - 2;
-            bool rx, ry;
-            probe_residues(tab, sp[u].d, sp[u].o, sp[u].k, rx, ry);
-            rx = rx || x[u] <= kLargestResiduePrime;
-            ry = ry || y[u] <= kLargestResiduePrime || sp[u].d == sp[u].o;
-            st[u] = st[u] && rx && ry;
-        }
-
-        // stage 2: PTP(2^k + q), q = d*o
-#pragma unroll
-        for (int u = 0; u < 2; ++u) ob[u] = (st[u] && x[u] <= a.M) ? map_byte(a.map, x[u]) : 0u;
-#pragma unroll
-        for (int u = 0; u < 2; ++u)
-            if (st[u]) st[u] = x[u] <= a.M ? byte_bit(ob[u], x[u]) : mr_is_prime(x[u]);
-
-        // stage 3: PTP(d + o*2^k) unless d == o
-        bool need2[2];
-#pragma unroll
-        for (int u = 0; u < 2; ++u) need2[u] = st[u] && sp[u].d != sp[u].o;
-#pragma unroll
-        for (int u = 0; u < 2; ++u) ob[u] = (need2[u] && y[u] <= a.M) ? map_byte(a.map, y[u]) : 0u;
-#pragma unroll
-        for (int u = 0; u < 2; ++u)
-            if (need2[u]) st[u] = y[u] <= a.M ? byte_bit(ob[u], y[u]) : mr_is_prime(y[u]);
-
-        // survivors wait in the queue for AllButOne (:638)
-#pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            const uint64_t cm = __ballot(st[u]);
-            if (st[u]) queue[qn + __popcll(cm & lt)] = make_uint2(sp[u].d | (sp[u].k << kDBits), sp[u].o);
-            qn += __popcll(cm);
-        }
-        if (qn == 0) continue;
+        const bool flush = base + 128 <= a.eager || base + 128 >= total;
+        if (pn < 128 && !(flush && pn > 0)) continue;
         wave_sync_lds();
-        const bool eager = base + 128 <= a.eager;
-        while (qn >= 64 || (eager && qn > 0)) {
-            const int nb = qn < 64 ? qn : 64;
-            drain(a, queue, qn - nb, nb, lane, pr, cnt, te, to);
-            qn -= nb;
-            if (cnt > 1) return 0;
+        while (pn >= 128 || (flush && pn > 0)) {
+            const int nb = pn < 128 ? pn : 128;
+            probe_batch(a, pq, pn - nb, nb, lane, lt, queue, qn);
+            pn -= nb;
+            wave_sync_lds();
+            while (qn >= 64 || (flush && qn > 0)) {
+                const int na = qn < 64 ? qn : 64;
+                drain(a, queue, qn - na, na, lane, pr, cnt, te, to);
+                qn -= na;
+                if (cnt > 1) return 0;
+            }
+            wave_sync_lds();
         }
     }
     wave_sync_lds();
@@ -602,6 +621,7 @@ __global__ void zero_u32(uint32_t *p, uint32_t n)
 __global__ __launch_bounds__(kBlock, 5) void search_sums(SearchArgs a)
 {
     __shared__ uint2 queues[kWavesPerBlock][kQueueCap];
+    __shared__ uint2 probes[kWavesPerBlock][kProbeCap];
     __shared__ uint2 segs[kWavesPerBlock][kMaxSegs];
     __shared__ uint32_t primes[kBlock][kMaxPrimes];
     __shared__ uint32_t tab[kTabWords];
@@ -617,7 +637,7 @@ __global__ __launch_bounds__(kBlock, 5) void search_sums(SearchArgs a)
         for (uint32_t idx = base; idx < end; ++idx) {
             const uint32_t slotIdx = a.nsums - 1 - idx;  // largest sums first
             const uint32_t s = sum_of_slot(a, slotIdx);
-            const uint32_t res = solve_sum_wave(a, tab, s, lane, queues[w], segs[w], primes[threadIdx.x]);
+            const uint32_t res = solve_sum_wave(a, tab, s, lane, queues[w], probes[w], segs[w], primes[threadIdx.x]);
             if (lane == 0) {
                 if (!a.bySum) a.slots[slotIdx] = res;
                 else if (s >= a.sumBase && ((s - a.sumBase) >> 1) < a.slotCap) a.slots[(s - a.sumBase) >> 1] = res;
