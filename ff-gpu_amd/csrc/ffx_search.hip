@@ -17,23 +17,23 @@
 // {2^k*d : d|q, d>1}, has !PTP(f + q*2^k/f)  (:424-477).
 // The count of valid splits, its early exit at two and the unique split's
 // terms do not depend on the order splits are visited in, so the 64 lanes
-// of a wavefront visit 64 splits per round and count with ballots.
+// of a wavefront visit splits in parallel and count with ballots.
 //
 // Per-sum work differs by five orders of magnitude (most sums stop after a
 // few hundred splits; emitted sums visit all ~s/4 splits), so waves pull sums
 // from one atomic counter, largest sums first.  Splits are visited k
 // descending as the reference does (it stops far earlier on non-solutions
-// than k ascending), flattened over all k so every lane is busy, two rounds
-// of 64 in flight per wave.  A probe value with a prime factor in 3..13 is
-// rejected by a residue table in LDS before it costs a random map read; the
-// residues of both probe values of a split come from those of d, o and 2^k,
-// and both are filtered before the split's first read.
-// Splits that pass both PTP probes go to a per-wave LDS queue; while a sum is
-// young every queued candidate is resolved at once (early exit), later they
-// are resolved 64 at a time, one per lane, so the rare factor-pair
-// enumerations do not serialize the probe rounds.  Terms d, o < s are
-// factored with a smallest-prime-factor table, so no 64-bit trial division
-// runs on the device.
+// than k ascending), flattened over all k so every lane is busy; Power2Odd
+// lists only the d its family admits.  Every split first goes through the
+// filters that need no map read: the family test and a residue filter (a
+// probe value with a prime factor in 3..23 is composite; the residues of
+// both probe values come from those of d, o and 2^k).  The few survivors
+// are compacted into a per-wave LDS probe queue and probed 128 at a time, so
+// the random map reads run on full waves; splits that pass both PTP probes
+// go to a second queue for AllButOne, resolved 64 at a time, one per lane.
+// While a sum is young both queues are emptied every round (early exit at
+// two valid splits).  Terms d, o < s are factored with a smallest-prime-
+// factor table, so no 64-bit trial division runs on the device.
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -47,9 +47,16 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / 64;
 constexpr int kQueueCap = 192;     // < 64 left after a drain + 128 from one probe batch
-constexpr int kProbeCap = 256;     // < 128 left after the batches + 128 from one round
+// Four splits per lane per filter round (measured: 26.4 ms against 28.8 with
+// two at the same occupancy, 36 with one).
+#ifndef FFX_ROUND_SPLITS
+#define FFX_ROUND_SPLITS 4
+#endif
+constexpr int kRoundSplits = FFX_ROUND_SPLITS;  // splits per lane per filter round
+constexpr int kProbeCap = 128 + 64 * kRoundSplits;  // < 128 left after the batches + one round
+static_assert(kRoundSplits >= 1 && kRoundSplits <= 4, "1..4 splits per lane per round");
 constexpr uint32_t kNeedO = 1u << 31;  // probe entry: o must be prime too (Power2Even, 3 !| d*o)
-constexpr uint32_t kGrab = 1;      // default sums taken per atomic (the autotuned choice)
+constexpr uint32_t kGrab = 2;      // default sums taken per atomic (the autotuned choice)
 constexpr int kMaxPrimes = 12;     // distinct odd primes of q < 3^31 (q <= s^2/16, s < 2^26)
 constexpr uint32_t kDBits = 27;    // queue entry: d | k << 27
 constexpr int kMaxSegs = 32;
@@ -57,7 +64,7 @@ constexpr int kMaxSegs = 32;
 // candidate is resolved after each step (most sums stop within a few hundred
 // splits); beyond it candidates are resolved 64 at a time.
 constexpr uint32_t kEagerSplits = 4096;
-constexpr uint32_t kBlocksPerCU = 8;  // default persistent grid: CUs x this
+constexpr uint32_t kBlocksPerCU = 4;  // default persistent grid: CUs x this (all resident)
 
 // Residue filter: a probe value larger than the largest prime of the moduli
 // in use that shares a factor with one of them is composite without touching
@@ -67,11 +74,11 @@ constexpr uint32_t kBlocksPerCU = 8;  // default persistent grid: CUs x this
 // the 32-bit residues of d, o and 2^k with 32-bit products.
 constexpr uint32_t kMods[5] = {3 * 5 * 7 * 11 * 13, 17 * 19 * 23, 29 * 31, 37 * 41, 43 * 47};
 constexpr uint32_t kModTopPrime[5] = {13, 23, 31, 41, 47};
-// One modulus measured fastest on MI355X: each further modulus costs more
-// integer issue than the map reads it saves ([5, 2^20) search: 40.5 ms with
-// one modulus, 41.7 with two, 43.7 with a third).
+// Two moduli measured fastest on MI355X with the probe queue ([5, 2^20)
+// search: 26.4 ms with one or two, 27.9 with three): each further modulus
+// costs more integer issue on every split than the probes it removes save.
 #ifndef FFX_RESIDUE_MODULI
-#define FFX_RESIDUE_MODULI 1
+#define FFX_RESIDUE_MODULI 2
 #endif
 constexpr int kNMods = FFX_RESIDUE_MODULI;
 static_assert(kNMods >= 1 && kNMods <= 5, "1..5 residue moduli");
@@ -535,7 +542,7 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
     }
     wave_sync_lds();
 
-    // Each round: 128 splits (two per lane) through the filters that need no
+    // Each round: 64 * kRoundSplits splits through the filters that need no
     // map read -- the family test of Power2Odd/Power2Even (:578, :619) and
     // the residue filter of both probe values (and of o where Power2Even
     // needs it prime); the few survivors are compacted into the probe queue
@@ -543,10 +550,10 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
     // the sum is young, queues are emptied every round (early exit at two
     // valid splits).
     int hint = 0, pn = 0;
-    for (uint32_t base = 0; base < total; base += 128) {
-        int segOf[2];
+    for (uint32_t base = 0; base < total; base += 64 * kRoundSplits) {
+        int segOf[kRoundSplits];
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
+        for (int u = 0; u < kRoundSplits; ++u) {
             const uint32_t g = base + 64u * u + lane;
             const bool live = g < total;
             int j = hint;
@@ -572,8 +579,8 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
             if (keep) pq[pn + __popcll(km & lt)] = make_uint2(d | (k << kDBits), o | (needO ? kNeedO : 0u));
             pn += __popcll(km);
         }
-        hint = __shfl(segOf[1], 63);
-        const bool flush = base + 128 <= a.eager || base + 128 >= total;
+        hint = __shfl(segOf[kRoundSplits - 1], 63);
+        const bool flush = base + 64 * kRoundSplits <= a.eager || base + 64 * kRoundSplits >= total;
         if (pn < 128 && !(flush && pn > 0)) continue;
         wave_sync_lds();
         while (pn >= 128 || (flush && pn > 0)) {
@@ -615,10 +622,12 @@ __global__ void zero_u32(uint32_t *p, uint32_t n)
     if (i < n) p[i] = 0;
 }
 
-// 5 blocks of 4 waves per CU: the register budget that measured fastest
-// (more waves spill the AB1/Miller-Rabin paths, fewer leave HBM latency
-// exposed).
-__global__ __launch_bounds__(kBlock, 5) void search_sums(SearchArgs a)
+// 4 blocks of 4 waves per CU: 128 registers per lane, no spills, and the
+// probe and AllButOne queues of four waves fit four blocks' LDS.
+#ifndef FFX_SEARCH_OCCUPANCY
+#define FFX_SEARCH_OCCUPANCY 4
+#endif
+__global__ __launch_bounds__(kBlock, FFX_SEARCH_OCCUPANCY) void search_sums(SearchArgs a)
 {
     __shared__ uint2 queues[kWavesPerBlock][kQueueCap];
     __shared__ uint2 probes[kWavesPerBlock][kProbeCap];
