@@ -102,8 +102,10 @@ __global__ __launch_bounds__(kThreads) void sieve_segments(SieveArgs a)
 {
     constexpr uint32_t kSegWords = kSegBytes / 4;
     constexpr uint32_t kSegOdds = kSegBytes * 8;  // odd values per segment
+    constexpr uint32_t kCoopMax = kSegOdds / 64 / 4;  // > the number of odd primes below kSegOdds/64
     __shared__ uint32_t seg[kSegWords];
     __shared__ uint32_t pat[kPatWords];
+    __shared__ uint32_t coopNext[kCoopMax];  // first strike of each cooperative prime, from the segment's start
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
@@ -115,9 +117,15 @@ __global__ __launch_bounds__(kThreads) void sieve_segments(SieveArgs a)
     // segment, so no division runs inside the loops.
     uint32_t period[kGroups], base[kGroups], stepFill[kGroups], stepSeg[kGroups], cur[kGroups];
     uint32_t off = 0;
+    // Each workgroup sieves a contiguous run of segments, so a cooperative
+    // prime's first strike in the next segment follows from this one's
+    // (kept in LDS) instead of a 64-bit division per segment.
+    const uint64_t perBlock = (a.segCount + gridDim.x - 1) / gridDim.x;
+    const uint64_t runLo = (uint64_t)blockIdx.x * perBlock;
+    const uint64_t runHi = runLo + perBlock < a.segCount ? runLo + perBlock : a.segCount;
     {
-        const uint64_t w0 = (a.segFirst + blockIdx.x) * (uint64_t)kSegWords + tid;
-        const uint64_t segStride = (uint64_t)gridDim.x * kSegWords;
+        const uint64_t w0 = (a.segFirst + runLo) * (uint64_t)kSegWords + tid;
+        const uint64_t segStride = kSegWords;
 #pragma unroll
         for (int g = 0; g < kGroups; ++g) {
             period[g] = group_period(g);
@@ -130,7 +138,7 @@ __global__ __launch_bounds__(kThreads) void sieve_segments(SieveArgs a)
     }
     __syncthreads();
 
-    for (uint64_t si = blockIdx.x; si < a.segCount; si += gridDim.x) {
+    for (uint64_t si = runLo; si < runHi; ++si) {
         const uint64_t segIdx = a.segFirst + si;
         const uint64_t j0 = segIdx * (uint64_t)kSegOdds;
         const uint64_t segByte0 = segIdx * (uint64_t)kSegBytes;
@@ -158,13 +166,33 @@ __global__ __launch_bounds__(kThreads) void sieve_segments(SieveArgs a)
         const uint64_t limit64 = a.oddLimit > j0 ? a.oddLimit - j0 : 0;
         const uint32_t limit = limit64 < kSegOdds ? (uint32_t)limit64 : kSegOdds;
         if (limit) {
+            // the same wave owns prime pi in every segment of the run, so
+            // coopNext[pi] is written and read by one wave only
             for (uint32_t pi = wave; pi < a.ncoop; pi += kWaves) {
                 const uint32_t p = a.primes[pi];
-                const uint64_t first = first_strike(j0, p, a.invp[pi]);
-                if (first >= limit) continue;
+                uint32_t first;
+                if (si == runLo) {
+                    const uint64_t f = first_strike(j0, p, a.invp[pi]);  // < (p*p-1)/2 < 2^32
+                    first = (uint32_t)f;
+                } else {
+                    first = coopNext[pi];
+                }
                 const uint32_t stride = 64u * p;
-                for (uint32_t j = (uint32_t)first + (uint32_t)lane * p; j < limit; j += stride)
+                for (uint32_t j = first + (uint32_t)lane * p; j < limit; j += stride)
                     atomicOr(&seg[j >> 5], 1u << (j & 31));
+                // first strike from the next segment's start: the first
+                // multiple at or after kSegOdds, n = ceil((kSegOdds-first)/p)
+                uint32_t next;
+                if (first >= kSegOdds) {
+                    next = first - kSegOdds;
+                } else {
+                    const uint32_t D = kSegOdds - first;  // < 2^20, so the float quotient is off by at most one
+                    uint32_t q = (uint32_t)((float)D * __builtin_amdgcn_rcpf((float)p));
+                    if (q * p > D) --q;
+                    if ((q + 1) * p <= D) ++q;
+                    next = first + (q + (q * p < D ? 1u : 0u)) * p - kSegOdds;
+                }
+                if (lane == 0) coopNext[pi] = next;
             }
             for (uint32_t pi = a.ncoop + tid; pi < a.nprimes; pi += kThreads) {
                 const uint32_t p = a.primes[pi];
@@ -314,6 +342,7 @@ static int launch(const ffx_sieve_plan *pl, uint8_t *dst, uint64_t dstBase, uint
     // primes with more than 64 strikes per segment strike wave-cooperatively
     a.ncoop = (uint32_t)(std::lower_bound(pl->hostPrimes.begin(), pl->hostPrimes.end(), (uint32_t)(segOdds / 64)) -
                          pl->hostPrimes.begin());
+    if (a.ncoop > segOdds / 256) return fail(FFX_EINVAL, "sieve: %u cooperative primes exceed the table", a.ncoop);
     a.patterns = pl->dPatterns;
     a.segFirst = byteLo / segBytes;
     a.segCount = (byteHi + segBytes - 1) / segBytes - a.segFirst;
