@@ -33,7 +33,7 @@ CASES = [
     # (binary, args, PRIMEMEM of that binary)
     ("ff_seg_cap16g", ["2097101", "2097200"], 16 << 30),
     ("ff_seg_cap16g", ["2096001", "2099000"], 16 << 30),
-    ("ff_seg", ["3000001", "3000100"], None),
+    ("ff_seg", ["3000001", "3002000"], None),
 ]
 
 
