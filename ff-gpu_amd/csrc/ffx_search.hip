@@ -141,11 +141,62 @@ __device__ __forceinline__ uint64_t mulmod52(uint64_t a, uint64_t b, uint64_t n)
     return (uint64_t)r;
 }
 
+// a*b mod n for a, b < n < 2^50, all exact integers held in doubles: h is
+// the rounded product and l = a*b - h exactly (one fused multiply-add); the
+// quotient floor(h/n) from the reciprocal is off by at most one (relative
+// error below 2^-51 on a quotient below 2^50), and h - q*n + l is an integer
+// below 2^52 in magnitude, so every step is exact and one correction each
+// way lands in [0, n).
+__device__ __forceinline__ double mulmod_f64(double a, double b, double n, double ninv)
+{
+    const double h = a * b;
+    const double l = fma(a, b, -h);
+    const double q = floor(h * ninv);
+    double r = fma(-q, n, h) + l;
+    if (r < 0) r += n;
+    if (r >= n) r -= n;
+    return r;
+}
+
+// Miller-Rabin for odd 17 <= n < 2^50 in double-precision arithmetic
+// (same bases and base counts as mr_is_prime).
+__device__ bool mr_is_prime_f64(uint64_t n)
+{
+    const double nd = (double)n, ninv = 1.0 / nd, mone = nd - 1.0;
+    const uint32_t r = (uint32_t)__builtin_ctzll(n - 1);
+    const uint64_t d = (n - 1) >> r;
+    const int nb = n < 3215031751ull ? 4 : n < 2152302898747ull ? 5 : n < 3474749660383ull ? 6 : 7;
+    const double bases[7] = {2, 3, 5, 7, 11, 13, 17};
+    for (int bi = 0; bi < nb; ++bi) {
+        double b = bases[bi], y = 1.0;
+        for (uint64_t e = d; e; e >>= 1) {
+            if (e & 1) y = mulmod_f64(y, b, nd, ninv);
+            b = mulmod_f64(b, b, nd, ninv);
+        }
+        if (y == 1.0 || y == mone) continue;
+        bool pass = false;
+        for (uint32_t j = 1; j < r; ++j) {
+            y = mulmod_f64(y, y, nd, ninv);
+            if (y == mone) {
+                pass = true;
+                break;
+            }
+            if (y == 1.0) return false;
+        }
+        if (!pass) return false;
+    }
+    return true;
+}
+
 // Deterministic Miller-Rabin with ff_seg's bases {2,3,5,7,11,13,17} and
 // base count by magnitude (AskMillerRabin, segmentedSieve.C:364-387).
-// n odd, 17 <= n < 2^63.
+// n odd, 17 <= n < 2^63.  Below 2^50 (every probe value of a sum below
+// 2^26) the double-precision path runs: a modular product is about ten
+// fused floating-point operations against ~30 integer ones in Montgomery
+// form.
 __device__ __noinline__ bool mr_is_prime(uint64_t n)
 {
+    if (n < (1ull << 50)) return mr_is_prime_f64(n);
     uint64_t inv = n;  // n * n == 1 mod 8
 #pragma unroll
     for (int i = 0; i < 5; ++i) inv *= 2 - n * inv;

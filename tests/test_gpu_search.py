@@ -80,6 +80,26 @@ def test_slots_match_oracle_beyond_goldens(map_value):
     assert got == want
 
 
+def test_slots_match_oracle_up_to_the_largest_sum():
+    # sums near 2^23 (the served workload's top), 2^25 and FFX_MAX_SUM
+    # (2^26 - 1) over a 16 MiB map: probe values reach s^2/4 < 2^50, the
+    # top of the double-precision Miller-Rabin range
+    ranges = [(2 ** 23 - 63, 2 ** 23 - 1), (2 ** 25 - 79, 2 ** 25 - 1), (2 ** 26 - 79, 2 ** 26 - 1)]
+    say("largest sums", ranges)
+    M = 2 ** 24 - 1
+    dmap = ffxpy.sieve_map(M)
+    plan = ffxpy.SearchPlan(2 ** 26 - 1)
+    got, sums = {}, []
+    for a, b in ranges:
+        for i, v in enumerate(ffxpy.search_slots(dmap, M, a, b, plan)):
+            s = (a | 1) + 2 * i
+            sums.append(s)
+            got[s] = int(v) & ffxpy.SLOT_LOW_MASK
+    want = _oracle_solve(sums, M)
+    assert got == want
+    assert sum(1 for v in want.values() if v) >= 4  # emitted sums among them
+
+
 def test_records_entry_point(c1_map):
     dmap, M = c1_map
     lib = ffxpy.load()
