@@ -39,6 +39,17 @@ def test_edge_cases_match_ff_seg():
         assert normalize(p.stdout.decode()) == c["stdout"], c["args"]
 
 
+def test_large_sums_match_ff_seg_fixtures():
+    # ff_seg's stdout beyond the goldens: across pmapSumLimit under a 16 GiB
+    # PRIMEMEM cap (three header lines, the search split in two calls) and
+    # near 3,000,000 uncapped (a 35 GB map)
+    for c in json.load(open(os.path.join(GOLDEN, "seg_large.json"))):
+        args = (["--primemem", c["primemem"]] if c["primemem"] else []) + c["args"]
+        p = ff_sieve(*args, timeout=600)
+        assert (p.returncode & 255) == (c["rc"] & 255), (c["args"], p.stderr.decode()[-500:])
+        assert normalize(p.stdout.decode()) == c["stdout"], c["args"]
+
+
 def test_map_cap_keeps_stdout():
     # a device map 256x smaller than ff_seg's: Miller-Rabin answers above it
     p = ff_sieve("--map-cap", 16777215, 5, 262143)

@@ -146,3 +146,23 @@ def test_oracle_solve_matches_goldens(ffref):
             assert lo == sol[s][0] and hi == s - lo, s
         else:
             assert lo == 0, s
+
+
+def test_oracle_solve_matches_ff_seg_beyond_the_goldens(ffref):
+    # ff_seg's own stdout (tests/golden/seg_large.json, written by
+    # make_large_fixtures.py from the reference program) for ranges across
+    # pmapSumLimit under a 16 GiB PRIMEMEM cap and near 3,000,000: the
+    # oracle's per-sum answers over a 1 GiB map, where Miller-Rabin decides
+    # most probes, give exactly the printed lines
+    import re
+    line = re.compile(r"^\s*\d+\) sum =\s*(\d+), product =\s*\d+,  low term =\s*(\d+)")
+    cases = json.load(open(os.path.join(GOLDEN, "seg_large.json")))
+    assert len(cases) == 3
+    for c in cases:
+        a, b = map(int, c["args"])
+        want = {int(m.group(1)): int(m.group(2)) for m in (line.match(l) for l in c["stdout"].split("\n")) if m}
+        sums = list(range(a | 1, b + 1, 2))
+        p = run_ffref(["--solve", "--map-cap", 2 ** 30 - 1], stdin="\n".join(map(str, sums)).encode())
+        got = {int(s): int(lo) for s, lo, _ in (l.split() for l in p.stdout.decode().split("\n") if l) if lo != "0"}
+        assert got == want, c["args"]
+        assert want  # every case prints solutions
