@@ -300,34 +300,45 @@ __device__ __forceinline__ uint32_t mod28(uint32_t n)
 // from residues; d, o, pk < 2^26 and m^2 + 2m < 2^28 keep every reduced
 // value below 2^28.
 template <uint32_t m>
-__device__ __forceinline__ void residue_pair(const uint32_t *t, uint32_t d, uint32_t o, uint32_t pk, bool &rx, bool &ry,
+__device__ __forceinline__ void residue_pair(const uint32_t *t, uint32_t d, uint32_t o, uint32_t pm, bool &rx, bool &ry,
                                              bool &ro)
 {
     static_assert((uint64_t)m * m + 2 * m < (1ull << 28), "residue products must stay below 2^28");
-    const uint32_t dm = mod28<m>(d), om = mod28<m>(o), pm = mod28<m>(pk);
+    const uint32_t dm = mod28<m>(d), om = mod28<m>(o);
     rx = rx & tab_bit(t, mod28<m>(dm * om + pm + m - 2));
     ry = ry & tab_bit(t, mod28<m>(dm + om * pm + m - 2));
     ro = ro & tab_bit(t, om);
 }
 
+// 2^k mod the first two moduli, packed (both below 2^16); the split
+// segment table carries it per k.
+__device__ __forceinline__ uint32_t pow2_residues(uint32_t k)
+{
+    const uint32_t pk = 1u << k;
+    uint32_t r = mod28<kMods[0]>(pk);
+    if constexpr (kNMods > 1) r |= mod28<kMods[1]>(pk) << 16;
+    return r;
+}
+
 template <int I>
-__device__ __forceinline__ void residue_all(const uint32_t *tab, uint32_t d, uint32_t o, uint32_t pk, bool &rx, bool &ry,
-                                            bool &ro)
+__device__ __forceinline__ void residue_all(const uint32_t *tab, uint32_t d, uint32_t o, uint32_t k, uint32_t pms,
+                                            bool &rx, bool &ry, bool &ro)
 {
     if constexpr (I < kNMods) {
-        residue_pair<kMods[I]>(tab + tab_offset(I), d, o, pk, rx, ry, ro);
-        residue_all<I + 1>(tab, d, o, pk, rx, ry, ro);
+        const uint32_t pm = I == 0 ? (pms & 0xFFFF) : I == 1 ? (pms >> 16) : mod28<kMods[I]>(1u << k);
+        residue_pair<kMods[I]>(tab + tab_offset(I), d, o, pm, rx, ry, ro);
+        residue_all<I + 1>(tab, d, o, k, pms, rx, ry, ro);
     }
 }
 
 // rx (ry, ro) false when X (Y, o) has a prime factor in
 // 3..kLargestResiduePrime; values above that prime are then composite.
-__device__ __forceinline__ void probe_residues(const uint32_t *tab, uint32_t d, uint32_t o, uint32_t k, bool &rx,
-                                               bool &ry, bool &ro)
+// pms = pow2_residues(k).
+__device__ __forceinline__ void probe_residues(const uint32_t *tab, uint32_t d, uint32_t o, uint32_t k, uint32_t pms,
+                                               bool &rx, bool &ry, bool &ro)
 {
     rx = ry = ro = true;
-    const uint32_t pk = 1u << k;
-    residue_all<0>(tab, d, o, pk, rx, ry, ro);
+    residue_all<0>(tab, d, o, k, pms, rx, ry, ro);
 }
 
 // ---------------------------------------------------------------- AB1
@@ -525,7 +536,7 @@ __device__ __forceinline__ void probe_batch(const SearchArgs &a, const uint2 *pq
 }
 
 __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const uint32_t *tab, uint32_t s, int lane,
-                                                   uint2 *queue, uint2 *pq, uint2 *segs, uint32_t *pr)
+                                                   uint2 *queue, uint2 *pq, uint4 *segs, uint32_t *pr)
 {
     // statement 1 (RunIt :806-807)
     if (is_prime_odd(a, s - 2)) return 0;
@@ -588,7 +599,7 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
             const uint32_t v = __shfl_up(incl, off);
             if (lane >= off) incl += v;
         }
-        if (lane < nseg) segs[lane] = make_uint2(incl - n, kj | (mode << 5) | (d0 << 8));
+        if (lane < nseg) segs[lane] = make_uint4(incl - n, kj | (mode << 5) | (d0 << 8), pow2_residues(kj), 0u);
         total = nseg ? __shfl(incl, nseg - 1) : 0;
     }
     wave_sync_lds();
@@ -600,6 +611,10 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
     // and probed 128 at a time, so the map reads run on full waves.  While
     // the sum is young, queues are emptied every round (early exit at two
     // valid splits).
+    // For s >= 4096 every split has X, Y > 47: if o >= 16, d*o >= 48 and
+    // o*2^k >= 64; otherwise d*2^k > 4080, so 2^k >= 64 or d >= 64.
+    static_assert(kLargestResiduePrime < 48, "tinySum bound");
+    const bool tinySum = s < 4096;
     int hint = 0, pn = 0;
     for (uint32_t base = 0; base < total; base += 64 * kRoundSplits) {
         int segOf[kRoundSplits];
@@ -611,21 +626,22 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
             if (live)
                 while (j + 1 < nseg && g >= segs[j + 1].x) ++j;
             segOf[u] = j;
-            const uint2 e = segs[j];
+            const uint4 e = segs[j];
             const uint32_t k = e.y & 31, mode = (e.y >> 5) & 3, i = g - e.x;
             const uint32_t d = (e.y >> 8) + (mode == 0 ? 2 * i : mode == 1 ? 6 * i : 6 * (i >> 1) + 2 * (i & 1));
             const uint32_t o = live ? s - (d << k) : 1;
-            const bool m3 = (d % 3 == 0) || (o % 3 == 0);
-            const bool odd = k & 1;
-            const bool needO = !odd && !m3;  // Power2Even without 3 | d*o: o must be prime
+            // Power2Even without 3 | d*o: o must be prime (Power2Odd lists
+            // hold only d with 3 !| d*o)
+            const bool needO = !(k & 1) && d % 3 != 0 && o % 3 != 0;
             bool rx, ry, ro;
-            probe_residues(tab, d, o, k, rx, ry, ro);
-            const uint64_t x = (1ull << k) + (uint64_t)d * o - 2;
-            const uint64_t y = (uint64_t)d + ((uint64_t)o << k) - 2;
-            rx = rx || x <= kLargestResiduePrime;
-            ry = ry || y <= kLargestResiduePrime || d == o;
+            probe_residues(tab, d, o, k, e.z, rx, ry, ro);
+            if (tinySum) {  // uniform: only sums below 4096 have probe values <= the largest residue prime
+                rx = rx || (1ull << k) + (uint64_t)d * o - 2 <= kLargestResiduePrime;
+                ry = ry || (uint64_t)d + ((uint64_t)o << k) - 2 <= kLargestResiduePrime;
+            }
+            ry = ry || d == o;
             ro = ro || o <= kLargestResiduePrime;
-            const bool keep = live && (odd ? !m3 : true) && rx && ry && (!needO || ro);
+            const bool keep = live && rx && ry && (!needO || ro);
             const uint64_t km = __ballot(keep);
             if (keep) pq[pn + __popcll(km & lt)] = make_uint2(d | (k << kDBits), o | (needO ? kNeedO : 0u));
             pn += __popcll(km);
@@ -682,7 +698,7 @@ __global__ __launch_bounds__(kBlock, FFX_SEARCH_OCCUPANCY) void search_sums(Sear
 {
     __shared__ uint2 queues[kWavesPerBlock][kQueueCap];
     __shared__ uint2 probes[kWavesPerBlock][kProbeCap];
-    __shared__ uint2 segs[kWavesPerBlock][kMaxSegs];
+    __shared__ uint4 segs[kWavesPerBlock][kMaxSegs];
     __shared__ uint32_t primes[kBlock][kMaxPrimes];
     __shared__ uint32_t tab[kTabWords];
     for (uint32_t i = threadIdx.x; i < kTabWords; i += kBlock) tab[i] = a.coprime[i];
