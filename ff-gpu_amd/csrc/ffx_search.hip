@@ -599,7 +599,9 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
             const uint32_t v = __shfl_up(incl, off);
             if (lane >= off) incl += v;
         }
-        if (lane < nseg) segs[lane] = make_uint4(incl - n, kj | (mode << 5) | (d0 << 8), pow2_residues(kj), 0u);
+        // d = d0 + a*(i >> h) + b*(i & 1): (a, b, h) = (2, 0, 0), (6, 0, 0), (6, 2, 1)
+        const uint32_t da = mode == 0 ? 2 : 6, db = mode == 2 ? 2 : 0, dh = mode == 2 ? 1 : 0;
+        if (lane < nseg) segs[lane] = make_uint4(incl - n, kj | (d0 << 8), pow2_residues(kj), da | (db << 4) | (dh << 8));
         total = nseg ? __shfl(incl, nseg - 1) : 0;
     }
     wave_sync_lds();
@@ -627,8 +629,8 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
                 while (j + 1 < nseg && g >= segs[j + 1].x) ++j;
             segOf[u] = j;
             const uint4 e = segs[j];
-            const uint32_t k = e.y & 31, mode = (e.y >> 5) & 3, i = g - e.x;
-            const uint32_t d = (e.y >> 8) + (mode == 0 ? 2 * i : mode == 1 ? 6 * i : 6 * (i >> 1) + 2 * (i & 1));
+            const uint32_t k = e.y & 31, i = g - e.x;
+            const uint32_t d = (e.y >> 8) + (e.w & 15) * (i >> (e.w >> 8)) + ((e.w >> 4) & 15) * (i & 1);
             const uint32_t o = live ? s - (d << k) : 1;
             // Power2Even without 3 | d*o: o must be prime (Power2Odd lists
             // hold only d with 3 !| d*o)
