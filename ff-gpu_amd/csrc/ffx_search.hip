@@ -536,7 +536,8 @@ __device__ __forceinline__ void probe_batch(const SearchArgs &a, const uint2 *pq
 }
 
 __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const uint32_t *tab, uint32_t s, int lane,
-                                                   uint2 *queue, uint2 *pq, uint4 *segs, uint32_t *pr)
+                                                   uint2 *queue, uint2 *pq, uint4 *segs, uint8_t *segOffs,
+                                                   uint32_t *pr)
 {
     // statement 1 (RunIt :806-807)
     if (is_prime_odd(a, s - 2)) return 0;
@@ -566,42 +567,65 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
     // Power2Odd and Power2Even splits, k descending as the reference visits
     // them (:570-572, :611-613), both families interleaved by k, flattened
     // into one index space: segment j is k = kTop - j with its d sequence.
-    // Power2Odd (k odd) needs 3 !| d and 3 !| o = s - d*2^k, i.e. d mod 3
-    // outside {0, 2s mod 3} (2^k = 2 mod 3): the segment lists only those d
-    // -- d = 1 mod 6 from 7 when s = 1 mod 3, d = 5 mod 6 when s = 2 mod 3,
-    // d = 5, 7, 11, 13, ... when 3 | s -- so no lane visits a split its
-    // family rejects.  Power2Even (k even) lists every odd d >= 3.
+    // Each segment lists only the residue classes of d mod 30 that can hold
+    // a valid split, decided from d, o = s - d*2^k and 2^k modulo 15:
+    //  * Power2Odd (k odd) needs 3 !| d and 3 !| o (its family, :578);
+    //  * both probe values X = 2^k + d*o - 2 and Y = d + o*2^k - 2 must be
+    //    prime, so neither may be divisible by 3 or 5 -- for s >= 4096 both
+    //    exceed 47 (see tinySum), and Y is not tested for the one split with
+    //    d = o, whose class is kept whole.
+    // That drops ~60% of the splits before any per-split work.
+    // For s >= 4096 every split has X, Y > 47: if o >= 16, d*o >= 48 and
+    // o*2^k >= 64; otherwise d*2^k > 4080, so 2^k >= 64 or d >= 64.
+    static_assert(kLargestResiduePrime < 48, "tinySum bound");
+    const bool tinySum = s < 4096;
     const uint32_t t3 = (s - 3) / 3;
     const int kTop = t3 >= 4 ? 31 - __clz(t3) : 0;  // largest k with 3*2^k <= s-3
     const int nseg = kTop >= 2 ? kTop - 1 : 0;
-    const uint32_t s3 = s % 3;
     uint32_t total = 0;
     {
-        uint32_t n = 0, kj = 0, d0 = 0, mode = 0;
+        constexpr uint32_t kCoprime15 = 0x6996u;  // r mod 15 coprime to 15: 1,2,4,7,8,11,13,14
+        constexpr uint32_t kNot3 = 0x6db6u;       // r mod 15 not divisible by 3
+        uint32_t n = 0, kj = 0, c = 0, skip = 0;
         if (lane < nseg) {
             kj = (uint32_t)(kTop - lane);
             const uint32_t D = (s - 3) >> kj;  // d*2^k < s-2  <=>  d <= (s-3)>>k
-            if (!(kj & 1)) {
-                d0 = 3;
-                n = D >= d0 ? (D - d0) / 2 + 1 : 0;
-            } else if (s3 != 0) {
-                mode = 1;
-                d0 = s3 == 1 ? 7 : 5;
-                n = D >= d0 ? (D - d0) / 6 + 1 : 0;
-            } else {
-                mode = 2;
-                d0 = 5;
-                n = D >= 5 ? 2 * ((D - 5) / 6) + ((D - 5) % 6 >= 2 ? 2 : 1) : 0;
+            const uint32_t dmin = (kj & 1) ? 5 : 3;
+            const uint32_t pk = (1u << kj) % 15, s15 = s % 15;
+            int dstarT = -1;  // class of the d = o split, if any
+            const uint32_t m = (1u << kj) + 1;
+            if (s % m == 0) {
+                const uint32_t ds = s / m;
+                if ((ds & 1) && ds >= dmin && ds <= D) dstarT = (int)((ds % 30) >> 1);
             }
+            const uint32_t P = (D + 1) / 30, rem = D - 30 * P;
+            uint32_t le = 0;
+#pragma unroll
+            for (uint32_t t = 0; t < 15; ++t) {
+                const uint32_t dr = (2 * t + 1) % 15;
+                const uint32_t orr = (s15 + 225 - dr * pk) % 15;
+                const uint32_t xr = (pk + dr * orr + 13) % 15, yr = (dr + orr * pk + 13) % 15;
+                bool ok = !(kj & 1) || (((kNot3 >> dr) & (kNot3 >> orr)) & 1);
+                if (!tinySum) ok = ok && ((kCoprime15 >> xr) & 1) && (((kCoprime15 >> yr) & 1) || (int)t == dstarT);
+                if (ok) {
+                    segOffs[lane * 16 + c] = (uint8_t)(2 * t + 1);
+                    ++c;
+                    skip += 2 * t + 1 < dmin;
+                    le += 2 * t + 1 <= rem;
+                }
+            }
+            n = (c && D >= dmin) ? c * P + le - skip : 0;
         }
         uint32_t incl = n;
         for (int off = 1; off < 64; off <<= 1) {
             const uint32_t v = __shfl_up(incl, off);
             if (lane >= off) incl += v;
         }
-        // d = d0 + a*(i >> h) + b*(i & 1): (a, b, h) = (2, 0, 0), (6, 0, 0), (6, 2, 1)
-        const uint32_t da = mode == 0 ? 2 : 6, db = mode == 2 ? 2 : 0, dh = mode == 2 ? 1 : 0;
-        if (lane < nseg) segs[lane] = make_uint4(incl - n, kj | (d0 << 8), pow2_residues(kj), da | (db << 4) | (dh << 8));
+        // index i of the segment: ii = i + skip, d = 30*(ii / c) + offs[ii % c]
+        // (ii < 2^24; for 2 <= c <= 15 the quotient is one high multiply by
+        // ceil(2^32 / c), exact since ii * (magic*c - 2^32) < 2^32)
+        const uint32_t magic = c >= 2 ? (uint32_t)(((1ull << 32) + c - 1) / c) : 0u;
+        if (lane < nseg) segs[lane] = make_uint4(incl - n, kj | (c << 8) | (skip << 16), pow2_residues(kj), magic);
         total = nseg ? __shfl(incl, nseg - 1) : 0;
     }
     wave_sync_lds();
@@ -613,10 +637,6 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
     // and probed 128 at a time, so the map reads run on full waves.  While
     // the sum is young, queues are emptied every round (early exit at two
     // valid splits).
-    // For s >= 4096 every split has X, Y > 47: if o >= 16, d*o >= 48 and
-    // o*2^k >= 64; otherwise d*2^k > 4080, so 2^k >= 64 or d >= 64.
-    static_assert(kLargestResiduePrime < 48, "tinySum bound");
-    const bool tinySum = s < 4096;
     int hint = 0, pn = 0;
     for (uint32_t base = 0; base < total; base += 64 * kRoundSplits) {
         int segOf[kRoundSplits];
@@ -629,8 +649,9 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
                 while (j + 1 < nseg && g >= segs[j + 1].x) ++j;
             segOf[u] = j;
             const uint4 e = segs[j];
-            const uint32_t k = e.y & 31, i = g - e.x;
-            const uint32_t d = (e.y >> 8) + (e.w & 15) * (i >> (e.w >> 8)) + ((e.w >> 4) & 15) * (i & 1);
+            const uint32_t k = e.y & 31, c = (e.y >> 8) & 15, ii = live ? g - e.x + (e.y >> 16) : 0u;
+            const uint32_t q = c == 1 ? ii : __umulhi(ii, e.w);
+            const uint32_t d = 30 * q + segOffs[j * 16 + (ii - q * c)];
             const uint32_t o = live ? s - (d << k) : 1;
             // Power2Even without 3 | d*o: o must be prime (Power2Odd lists
             // hold only d with 3 !| d*o)
@@ -701,6 +722,7 @@ __global__ __launch_bounds__(kBlock, FFX_SEARCH_OCCUPANCY) void search_sums(Sear
     __shared__ uint2 queues[kWavesPerBlock][kQueueCap];
     __shared__ uint2 probes[kWavesPerBlock][kProbeCap];
     __shared__ uint4 segs[kWavesPerBlock][kMaxSegs];
+    __shared__ uint8_t segOffs[kWavesPerBlock][kMaxSegs * 16];  // allowed d mod 30 per segment
     __shared__ uint32_t primes[kBlock][kMaxPrimes];
     __shared__ uint32_t tab[kTabWords];
     for (uint32_t i = threadIdx.x; i < kTabWords; i += kBlock) tab[i] = a.coprime[i];
@@ -715,7 +737,7 @@ __global__ __launch_bounds__(kBlock, FFX_SEARCH_OCCUPANCY) void search_sums(Sear
         for (uint32_t idx = base; idx < end; ++idx) {
             const uint32_t slotIdx = a.nsums - 1 - idx;  // largest sums first
             const uint32_t s = sum_of_slot(a, slotIdx);
-            const uint32_t res = solve_sum_wave(a, tab, s, lane, queues[w], probes[w], segs[w], primes[threadIdx.x]);
+            const uint32_t res = solve_sum_wave(a, tab, s, lane, queues[w], probes[w], segs[w], segOffs[w], primes[threadIdx.x]);
             if (lane == 0) {
                 if (!a.bySum) a.slots[slotIdx] = res;
                 else if (s >= a.sumBase && ((s - a.sumBase) >> 1) < a.slotCap) a.slots[(s - a.sumBase) >> 1] = res;
