@@ -23,11 +23,13 @@
 // few hundred splits; emitted sums visit all ~s/4 splits), so waves pull sums
 // from one atomic counter, largest sums first.  Splits are visited k
 // descending as the reference does (it stops far earlier on non-solutions
-// than k ascending), flattened over all k so every lane is busy; Power2Odd
-// lists only the d its family admits.  Every split first goes through the
-// filters that need no map read: the family test and a residue filter (a
-// probe value with a prime factor in 3..23 is composite; the residues of
-// both probe values come from those of d, o and 2^k).  The few survivors
+// than k ascending), flattened over all k so every lane is busy; each k
+// lists only the classes of d mod 30 that can hold a valid split (the
+// Power2Odd family, both probe values coprime to 15).  Every listed split
+// first goes through the filters that need no map read: the Power2Even
+// family test and a residue filter (a probe value with a prime factor in
+// 7..31 is composite; the residues of both probe values come from those of
+// d, o and 2^k).  The few survivors
 // are compacted into a per-wave LDS probe queue and probed 128 at a time, so
 // the random map reads run on full waves; splits that pass both PTP probes
 // go to a second queue for AllButOne, resolved 64 at a time, one per lane.
@@ -74,11 +76,12 @@ constexpr uint32_t kBlocksPerCU = 4;  // default persistent grid: CUs x this (al
 // the 32-bit residues of d, o and 2^k with 32-bit products.
 constexpr uint32_t kMods[5] = {3 * 5 * 7 * 11 * 13, 17 * 19 * 23, 29 * 31, 37 * 41, 43 * 47};
 constexpr uint32_t kModTopPrime[5] = {13, 23, 31, 41, 47};
-// Two moduli measured fastest on MI355X with the probe queue ([5, 2^20)
-// search: 26.4 ms with one or two, 27.9 with three): each further modulus
-// costs more integer issue on every split than the probes it removes save.
+// Three moduli measured fastest on MI355X with the mod-30 split classes
+// ([5, 2^20) search: 23.6 ms with one, 20.2 with two, 19.7 with three to
+// five; [5, 2^21): 69.7 ms with three).  The classes already exclude 3 and
+// 5; the tables add 7..31.
 #ifndef FFX_RESIDUE_MODULI
-#define FFX_RESIDUE_MODULI 2
+#define FFX_RESIDUE_MODULI 3
 #endif
 constexpr int kNMods = FFX_RESIDUE_MODULI;
 static_assert(kNMods >= 1 && kNMods <= 5, "1..5 residue moduli");
