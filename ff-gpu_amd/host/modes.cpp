@@ -778,7 +778,7 @@ int runAutotune(const Config &cfg, const AutotuneOptions &opt)
         return fail("search reference", rc);
     double defaultMs = 0;
     for (uint32_t grab : {1u, 2u, 4u})
-        for (uint32_t eager : {1024u, 4096u, 16384u})
+        for (uint32_t eager : {256u, 1024u, 4096u})
             for (uint32_t bpc : {4u, 5u, 6u, 8u}) {
                 if ((rc = ffx_search_plan_set(qp, FFX_SEARCH_GRAB, grab)) ||
                     (rc = ffx_search_plan_set(qp, FFX_SEARCH_EAGER_SPLITS, eager)) ||
