@@ -42,4 +42,28 @@ std::string bytesToHuman(uint64_t bytes)
     return buf;
 }
 
+uint64_t mapValueFor(uint64_t maxM, uint64_t cap)
+{
+    uint64_t M = maxM;
+    if (cap && cap < M) M = cap;
+    M = ((M + 1) & ~15ull) - 1;
+    return M < 15 ? 15 : M;
+}
+
+uint64_t uncappedMapValue(uint32_t L)
+{
+    const uint64_t mgp = ((uint64_t)L + 1) >> 1;
+    const uint64_t need = mgp * mgp / 4 + 2;
+    return ((need + 15) & ~15ull) - 1;
+}
+
+uint64_t serveMapValue(uint64_t backing, uint64_t fixedBytes, uint32_t maxSum, uint64_t mapCap)
+{
+    if (backing < fixedBytes + (64ull << 20)) return 0;
+    const uint64_t room = backing - fixedBytes;
+    uint64_t M = mapValueFor(uncappedMapValue(maxSum), mapCap);
+    if ((M >> 4) + 1 > room) M = mapValueFor(((room - 1) << 4) - 1, 0);
+    return M;
+}
+
 }  // namespace ffg

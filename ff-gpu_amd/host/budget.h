@@ -27,6 +27,22 @@ DeviceBudget computeBudget(uint64_t freeBytes, const Config &cfg, double fractio
 DeviceBudget shrinkFractionForAlloc(uint64_t freeBytes, const Config &cfg, double fraction);
 std::string bytesToHuman(uint64_t bytes);
 
+// Device map value for a map that must answer values up to maxM: maxM, or
+// the --map-cap bound when smaller (Miller-Rabin answers above the map), as
+// a whole number of map bytes (16*bytes - 1), at least 15.
+uint64_t mapValueFor(uint64_t maxM, uint64_t cap);
+
+// The map value that keeps the probes of sums up to L in the bitmap as far
+// as ff_seg's uncapped geometry does: ((L+1)>>1)^2/4 + 2, rounded up to a
+// whole map byte (reference/segmentedSieve.C:874-879).
+uint64_t uncappedMapValue(uint32_t L);
+
+// Admission of the serving loop's resident map against the budget: the
+// uncapped map for sums up to maxSum (bounded by mapCap), or the largest map
+// the backing holds after fixedBytes of other buffers.  0 when the backing
+// cannot hold the fixed buffers and 64 MiB of map.
+uint64_t serveMapValue(uint64_t backing, uint64_t fixedBytes, uint32_t maxSum, uint64_t mapCap);
+
 }  // namespace ffg
 
 #endif

@@ -175,6 +175,11 @@ int main(int argc, char **argv)
         for (uint32_t s = 1; s <= top + 6; s += 2) std::printf("%" PRId64 "\n", U.slotOf(s));
         return 0;
     }
+    if (cmd == "servemap" && argc >= 6) {
+        // serving admission: backing fixedBytes maxSum mapCap -> map value
+        std::printf("%" PRIu64 "\n", serveMapValue(u64(argv[2]), u64(argv[3]), (uint32_t)u64(argv[4]), u64(argv[5])));
+        return 0;
+    }
     if (cmd == "job" && argc >= 4) {
         // makeJob: clamped range, chunk list and evaluated odd-sum span
         const RangeJob j = makeJob((uint32_t)u64(argv[2]), (uint32_t)u64(argv[3]), argc >= 5 ? u64(argv[4]) : kSegPrimeMem);

@@ -48,16 +48,6 @@ int fail(const char *what, int rc)
     return 1;
 }
 
-// Device map value for a set of jobs: the largest ff_seg M among them, or
-// the --map-cap bound (Miller-Rabin answers above the map either way).
-uint64_t mapValueFor(uint64_t maxM, uint64_t cap)
-{
-    uint64_t M = maxM;
-    if (cap && cap < M) M = cap;
-    M = ((M + 1) & ~15ull) - 1;
-    return M < 15 ? 15 : M;
-}
-
 int applyTuning(const Config &cfg, ffx_sieve_plan *sp, ffx_search_plan *qp)
 {
     int rc = 0;
@@ -351,16 +341,6 @@ struct InFlight {
 
 }  // namespace
 
-// Map bound that keeps the probes of sums up to L in the bitmap as far as
-// ff_seg's own geometry does (uncapped: ((L+1)>>1)^2/4 + 2, rounded up to a
-// whole map byte, segmentedSieve.C:874-879).
-static uint64_t uncappedMapValue(uint32_t L)
-{
-    const uint64_t mgp = ((uint64_t)L + 1) >> 1;
-    const uint64_t need = mgp * mgp / 4 + 2;
-    return ((need + 15) & ~15ull) - 1;
-}
-
 int runServe(const Config &cfg, const ServeOptions &opt)
 {
     const std::vector<Request> reqs =
@@ -392,14 +372,12 @@ int runServe(const Config &cfg, const ServeOptions &opt)
     // resident map gets what the backing holds after the fixed buffers.
     // Probe values above the map are answered by Miller-Rabin, so the bound
     // only trades speed for memory.
-    if (dev.budget.backing < fixedBytes + (64ull << 20)) {
+    const uint64_t M = serveMapValue(dev.budget.backing, fixedBytes, maxSum, cfg.mapCap);
+    if (!M) {
         std::fprintf(stderr, "[ff_sieve] ERROR: budget %s cannot hold the serving buffers\n",
                      bytesToHuman(dev.budget.backing).c_str());
         return 1;
     }
-    const uint64_t mapRoom = dev.budget.backing - fixedBytes;
-    uint64_t M = mapValueFor(uncappedMapValue(maxSum), cfg.mapCap);
-    if ((M >> 4) + 1 > mapRoom) M = mapValueFor(((mapRoom - 1) << 4) - 1, 0);
     const uint64_t mapBytesMax = (M >> 4) + 1;
     std::fprintf(stderr,
                  "[ff_sieve] serve: %zu requests at %.0f/s, %llu requested odd sums up to %u; resident map up to "

@@ -224,3 +224,19 @@ def test_job_span_matches_oracle_chunks(ffref, a, b):
     assert chunks == ref
     odd = [s for lo, hi in ref for s in range(lo | 1, hi + 1, 2)]
     assert j["any"] == "1" and tuple(map(int, j["span"].split())) == (min(odd), max(odd))
+
+
+def test_serve_admission_against_the_budget():
+    # the serving loop's resident map: ff_seg's uncapped bound for the
+    # largest sum when the backing holds it (2^23 - 1: ((2^22)^2)/4 + 2 rounded
+    # up to a whole map byte), else the largest map the backing holds after
+    # the fixed buffers, else refusal; --map-cap bounds it
+    GiB, MiB = 1 << 30, 1 << 20
+    assert hc("servemap", 512 * GiB, 16 * MiB, 2 ** 23 - 1, 0).stdout.strip() == "4398046511119"
+    M = int(hc("servemap", 256 * GiB, 16 * MiB, 2 ** 23 - 1, 0).stdout)
+    room = 256 * GiB - 16 * MiB
+    assert room - 16 < (M >> 4) + 1 <= room and M % 16 == 15
+    assert hc("servemap", 64 * MiB, 16 * MiB, 2 ** 23 - 1, 0).stdout.strip() == "0"
+    assert hc("servemap", 512 * GiB, 16 * MiB, 2 ** 23 - 1, 2 ** 34 - 1).stdout.strip() == str(2 ** 34 - 1)
+    # small sums: the uncapped bound of 1048575 is ff_seg's own M for that leg
+    assert hc("servemap", 512 * GiB, 16 * MiB, 1048575, 0).stdout.strip() == "68719476751"
