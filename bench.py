@@ -15,7 +15,7 @@ graph-captured persistent launch) and the served workload (20000 seeded
 Poisson requests, p50/p99 latency beside sequential plain launches); their
 summaries are attached as "batched" and "served".
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c3] [--no-cpu] [--no-modes]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload sums2p20|sums2p21] [--no-cpu] [--no-modes]
 """
 import argparse
 import ctypes as C
@@ -37,14 +37,9 @@ HBM_PEAK_GBS = 8000.0
 
 WORKLOADS = {
     # name: (sumLimit, golden leg, description)
-    "c2": (1048575, 1048576, "plain search, odd sums [5, 2^20): one sieve pass (4 GiB map) + one search launch"),
-    "c3": (2097151, 2097152, "plain search, odd sums [5, 2^21): one sieve pass (16 GiB map) + one search launch"),
+    "sums2p20": (1048575, 1048576, "plain search, odd sums [5, 2^20): one sieve pass (4 GiB map) + one search launch"),
+    "sums2p21": (2097151, 2097152, "plain search, odd sums [5, 2^21): one sieve pass (16 GiB map) + one search launch"),
 }
-
-# Reference-order work of the search, counted by the oracle (ffref --counters)
-# on the same workload: bitmap probes the reference's CPU search makes.
-REF_PROBES = {"c2": None, "c3": None}
-
 
 def load_ref_work():
     p = os.path.join(ROOT, "profiles", "reference_work.json")
@@ -130,7 +125,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="sums2p20", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-timeout", type=float, default=150.0)
     ap.add_argument("--no-modes", action="store_true", help="skip the batched and served legs")
@@ -153,7 +148,7 @@ def main():
     lib = ffxpy.load()
     ffxpy.init(0)
     dev = ffxpy.device_info(0)
-    M = 68719476751 if args.workload == "c2" else 274877906959
+    M = 68719476751 if args.workload == "sums2p20" else 274877906959
     map_bytes = (M >> 4) + 1
     nodd = (sum_limit - 5) // 2 + 1
 

@@ -32,7 +32,7 @@ def c1_map():
 
 def test_c1_solutions_match_goldens(c1_map):
     dmap, M = c1_map
-    say("c1 search")
+    say("search of odd sums 5..65535")
     got = ffxpy.search(dmap, M, 5, 65535)
     want = [(s, lo, tags_to_flags(t)) for s, lo, t in golden_solutions() if s <= 65535]
     assert len(got) == len(want) == 2357
