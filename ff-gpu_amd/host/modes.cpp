@@ -16,6 +16,7 @@
 #include <sys/stat.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -213,28 +214,40 @@ int runBatch(const Config &cfg, const BatchOptions &opt)
     if (U.totalSlots && (rc = ffx_copy_d2h(slots.data(), dSlots, U.totalSlots * 4))) return fail("slot copy", rc);
     const double searchMs = *std::min_element(runs.begin(), runs.end());
 
-    // per-range stdout, each exactly a plain run's
+    // per-range stdout, each exactly a plain run's, formatted by 8 host
+    // threads taking ranges in turn
     const auto tE = Clock::now();
-    uint64_t lines = 0;
     if (!opt.outDir.empty()) mkdir(opt.outDir.c_str(), 0755);
-    std::string out;
-    for (size_t i = 0; i < jobs.size(); ++i) {
-        out.clear();
-        if (jobs[i].geom.fatal) out = fatalText(jobs[i]);
-        else
-            lines += emitJob(out, jobs[i], [&](uint32_t s) {
-                const int64_t k = U.slotOf(s);
-                return k < 0 ? 0u : slots[(size_t)k];
-            }, (uint64_t)(sieveWall * 1000), (uint64_t)(searchWall * 1000));
-        if (!opt.outDir.empty()) {
-            char name[64];
-            std::snprintf(name, sizeof name, "/range_%05zu.txt", i);
-            if (!writeFile(opt.outDir + name, out)) {
-                std::fprintf(stderr, "[ff_sieve] error: cannot write %s%s\n", opt.outDir.c_str(), name);
-                return 1;
+    std::atomic<size_t> nextRange{0};
+    std::atomic<uint64_t> lineCount{0};
+    std::atomic<bool> writeFailed{false};
+    auto emitter = [&] {
+        std::string out;
+        uint64_t nl = 0;
+        for (size_t i; (i = nextRange.fetch_add(1)) < jobs.size();) {
+            out.clear();
+            if (jobs[i].geom.fatal) out = fatalText(jobs[i]);
+            else
+                nl += emitJob(out, jobs[i], [&](uint32_t s) {
+                    const int64_t k = U.slotOf(s);
+                    return k < 0 ? 0u : slots[(size_t)k];
+                }, (uint64_t)(sieveWall * 1000), (uint64_t)(searchWall * 1000));
+            if (!opt.outDir.empty()) {
+                char name[64];
+                std::snprintf(name, sizeof name, "/range_%05zu.txt", i);
+                if (!writeFile(opt.outDir + name, out)) writeFailed = true;
             }
         }
+        lineCount += nl;
+    };
+    std::vector<std::thread> emitters;
+    for (int t = 0; t < 8; ++t) emitters.emplace_back(emitter);
+    for (auto &t : emitters) t.join();
+    if (writeFailed) {
+        std::fprintf(stderr, "[ff_sieve] error: cannot write the range files under %s\n", opt.outDir.c_str());
+        return 1;
     }
+    const uint64_t lines = lineCount;
     const double emitMs = msSince(tE);
     std::printf("{\"mode\": \"batch\", \"ranges\": %zu, \"requested_odd_sums\": %llu, \"distinct_odd_sums\": %llu, "
                 "\"intervals\": %zu, \"solution_lines\": %llu, \"map_value\": %llu, \"map_bytes\": %llu, "
