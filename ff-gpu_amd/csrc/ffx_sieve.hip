@@ -32,11 +32,17 @@
 
 namespace {
 
-// Defaults measured fastest on MI355X by ff_sieve --autotune (odd sums to
-// 2^20, 4 GiB map): 32 KiB segments, four 1024-thread workgroups per CU (two
-// resident at a time with the 40 KiB of patterns).
-constexpr uint32_t kDefaultSegBytes = 32768;  // LDS tile: map bytes per segment
-constexpr uint32_t kDefaultBlocksPerCU = 4;
+// Defaults measured on MI355X (ff_sieve --autotune and direct timing):
+// maps whose sieving primes stay below 2^19 (M < 2^38) are fastest with
+// 32 KiB segments and four 1024-thread workgroups per CU (two resident with
+// the 40 KiB of patterns): 4 GiB map 12.9 ms against 13.9 with 64 KiB.
+// Larger maps, where most primes strike a segment at most a few times and
+// each segment pays every prime's first-multiple computation, are faster
+// with 64 KiB segments, two per CU: 70 GB map 316 against 346 ms, 256 GiB
+// map 1.65 against 1.99 s.
+constexpr uint32_t kSmallMapSegBytes = 32768, kSmallMapBlocksPerCU = 4;
+constexpr uint32_t kLargeMapSegBytes = 65536, kLargeMapBlocksPerCU = 2;
+constexpr uint64_t kLargeMapFrom = 1ull << 38;
 constexpr int kThreads = 1024;
 constexpr int kWaves = kThreads / 64;
 
@@ -302,8 +308,9 @@ static int plan_build(ffx_sieve_plan **out, std::vector<uint32_t> &oddPrimes, ui
     pl->maxPrimeMapValue = M;
     pl->nprimes = (uint32_t)med.size();
     pl->hostPrimes = med;
-    pl->segBytes = kDefaultSegBytes;
-    pl->blocksPerCU = kDefaultBlocksPerCU;
+    const bool large = M >= kLargeMapFrom;
+    pl->segBytes = large ? kLargeMapSegBytes : kSmallMapSegBytes;
+    pl->blocksPerCU = large ? kLargeMapBlocksPerCU : kSmallMapBlocksPerCU;
     hipError_t e = hipGetDevice(&pl->device);
     int cus = 0;
     if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, pl->device);
