@@ -414,21 +414,27 @@ int runServe(const Config &cfg, const ServeOptions &opt)
         hIv[k] = (ffx_interval *)ffx_alloc_pinned((uint64_t)ivCap * sizeof(ffx_interval));
         if (!st[k] || !ev[k] || !dIv[k] || !hIv[k]) return fail("stream resources", FFX_ENOMEM);
     }
-    void *gst = ffx_stream_create(), *gev = ffx_event_create();
+    void *gst = ffx_stream_create();
     void *dMap = ffx_alloc((mapBytesMax + 255) & ~255ull);
     uint32_t *dSlots = (uint32_t *)ffx_alloc(nslots * 4);
     uint32_t *mirror = (uint32_t *)ffx_alloc_pinned(nslots * 4);
-    if (!gst || !gev || !dMap || !dSlots || !mirror) return fail("allocation", FFX_ENOMEM);
+    if (!gst || !dMap || !dSlots || !mirror) return fail("allocation", FFX_ENOMEM);
 
     // The first GiB of the map is sieved before the stream opens (small
-    // values are always probed); the rest grows on its own stream while
-    // batches run, each batch using the bytes finished when it launches.
+    // values are always probed).  The rest grows on demand on its own stream,
+    // queued in 16 GiB launches as soon as admitted requests need it, each
+    // with an event, so the finished prefix advances while batches run.  A
+    // batch waits until the map covers ff_seg's bound for its largest sum:
+    // a search over a partial map would decide the missing probe values by
+    // Miller-Rabin, several times slower than waiting for the sieve.
     const uint64_t growStep = 16ull << 30;
-    uint64_t mapBytesDone = std::min<uint64_t>(mapBytesMax, 1ull << 30), growTarget = 0, growths = 1;
+    uint64_t mapBytesDone = std::min<uint64_t>(mapBytesMax, 1ull << 30), growths = 1;
     if ((rc = ffx_sieve_map_range_async(sp, dMap, 0, mapBytesDone, gst)) || (rc = ffx_stream_sync(gst)))
         return fail("map", rc);
-    bool growing = false;
-    uint64_t wantBytes = mapBytesDone;
+    uint64_t wantBytes = mapBytesDone, issuedBytes = mapBytesDone;
+    std::deque<std::pair<void *, uint64_t>> growQ;  // (event, map bytes done when it fires)
+    std::vector<void *> growEvents;
+    auto bytesFor = [&](uint32_t L) { return (std::min(M, uncappedMapValue(L)) >> 4) + 1; };
 
     std::vector<double> doneAt(n, -1.0);
     Spans computed, claimed;  // claimed: computed, running or queued
@@ -476,32 +482,37 @@ int runServe(const Config &cfg, const ServeOptions &opt)
             for (const auto &m : missing) work.push_back(m);
             mergeInto(claimed, missing);
             waiting.push_back(next);
-            wantBytes = std::max(wantBytes, (std::min(M, uncappedMapValue(j.hi)) >> 4) + 1);
+            wantBytes = std::max(wantBytes, bytesFor(j.hi));
         }
         // map growth, concurrent with the batches
-        if (growing) {
-            const int q = ffx_event_query(gev);
-            if (q < 0) return fail("map growth", q);
-            if (q == 0) {
-                mapBytesDone = growTarget;
-                growing = false;
-                progress = true;
-            }
-        }
-        if (!growing && wantBytes > mapBytesDone) {
-            growTarget = std::min(wantBytes, mapBytesDone + growStep);
-            if ((rc = ffx_sieve_map_range_async(sp, dMap, mapBytesDone, growTarget, gst)) ||
-                (rc = ffx_event_record(gev, gst)))
+        while (issuedBytes < wantBytes) {
+            const uint64_t to = std::min(wantBytes, issuedBytes + growStep);
+            void *e = ffx_event_create();
+            if (!e) return fail("growth event", FFX_ENOMEM);
+            growEvents.push_back(e);
+            if ((rc = ffx_sieve_map_range_async(sp, dMap, issuedBytes, to, gst)) || (rc = ffx_event_record(e, gst)))
                 return fail("map growth", rc);
-            growing = true;
+            growQ.push_back({e, to});
+            issuedBytes = to;
             ++growths;
         }
-        // batches: the oldest queued sums, at most batchSums per launch
+        while (!growQ.empty()) {
+            const int q = ffx_event_query(growQ.front().first);
+            if (q < 0) return fail("map growth", q);
+            if (q != 0) break;
+            mapBytesDone = growQ.front().second;
+            growQ.pop_front();
+            progress = true;
+        }
+        // batches: the oldest queued sums, at most batchSums per launch, once
+        // the map covers them
         for (int k = 0; k < S && !work.empty(); ++k) {
             if (busy[k]) continue;
+            if (bytesFor(work.front().second) > mapBytesDone) break;
             std::vector<ffx_interval> iv;
             uint64_t total = 0;
-            while (!work.empty() && total < batchSums && iv.size() < ivCap) {
+            while (!work.empty() && total < batchSums && iv.size() < ivCap &&
+                   bytesFor(work.front().second) <= mapBytesDone) {
                 auto &f = work.front();
                 const uint64_t cnt = (f.second - f.first) / 2 + 1, take = std::min(cnt, batchSums - total);
                 ffx_interval x;
@@ -575,8 +586,11 @@ int runServe(const Config &cfg, const ServeOptions &opt)
     }
     pool.finish();
     const double makespan = now();
-    if (growing && (rc = ffx_event_sync(gev))) return fail("map growth", rc);
-    if (growing) mapBytesDone = growTarget;
+    if (!growQ.empty()) {
+        if ((rc = ffx_stream_sync(gst))) return fail("map growth", rc);
+        mapBytesDone = growQ.back().second;
+        growQ.clear();
+    }
     std::vector<double> lat(n);
     for (size_t i = 0; i < n; ++i) lat[i] = (doneAt[i] - reqs[i].arrivalS) * 1000.0;
     const double p50 = percentile(lat, 0.50), p99 = percentile(lat, 0.99);
@@ -651,7 +665,7 @@ int runServe(const Config &cfg, const ServeOptions &opt)
     ffx_free(dSlots);
     ffx_free(dMap);
     ffx_stream_destroy(gst);
-    ffx_event_destroy(gev);
+    for (void *e : growEvents) ffx_event_destroy(e);
     for (int k = 0; k < S; ++k) {
         ffx_search_plan_destroy(qp[k]);
         ffx_stream_destroy(st[k]);
