@@ -467,8 +467,9 @@ int runServe(const Config &cfg, const ServeOptions &opt)
     });
 
     size_t next = 0, completed = 0;
+    bool idleScan = false;  // the last scan found no queued sum the map covers
     while (completed < n) {
-        bool progress = false, admitted = false;
+        bool progress = false, admitted = false, grew = false;
         // admission: arrived requests queue the sums nothing has claimed
         for (const double t = now(); next < n && reqs[next].arrivalS <= t; ++next) {
             const RangeJob &j = jobs[next];
@@ -502,27 +503,38 @@ int runServe(const Config &cfg, const ServeOptions &opt)
             if (q != 0) break;
             mapBytesDone = growQ.front().second;
             growQ.pop_front();
-            progress = true;
+            progress = grew = true;
         }
-        // batches: the oldest queued sums, at most batchSums per launch, once
-        // the map covers them
-        for (int k = 0; k < S && !work.empty(); ++k) {
+        // batches: the oldest queued sums that the map covers, at most
+        // batchSums per launch (sums still waiting for the map stay queued
+        // without holding back the ones behind them)
+        if (admitted || grew) idleScan = false;
+        for (int k = 0; k < S && !work.empty() && !idleScan; ++k) {
             if (busy[k]) continue;
-            if (bytesFor(work.front().second) > mapBytesDone) break;
             std::vector<ffx_interval> iv;
             uint64_t total = 0;
-            while (!work.empty() && total < batchSums && iv.size() < ivCap &&
-                   bytesFor(work.front().second) <= mapBytesDone) {
-                auto &f = work.front();
-                const uint64_t cnt = (f.second - f.first) / 2 + 1, take = std::min(cnt, batchSums - total);
+            for (auto w = work.begin(); w != work.end() && total < batchSums && iv.size() < ivCap;) {
+                if (bytesFor(w->second) > mapBytesDone) {
+                    ++w;
+                    continue;
+                }
+                const uint64_t cnt = (w->second - w->first) / 2 + 1, take = std::min(cnt, batchSums - total);
                 ffx_interval x;
-                x.first = f.first;
-                x.last = f.first + 2 * (uint32_t)(take - 1);
+                x.first = w->first;
+                x.last = w->first + 2 * (uint32_t)(take - 1);
                 x.slotOffset = 0;
                 iv.push_back(x);
                 total += take;
-                if (take == cnt) work.pop_front();
-                else f.first = x.last + 2;
+                if (take == cnt) {
+                    w = work.erase(w);
+                } else {
+                    w->first = x.last + 2;
+                    ++w;
+                }
+            }
+            if (iv.empty()) {
+                idleScan = true;
+                break;
             }
             std::sort(iv.begin(), iv.end(), [](const ffx_interval &a, const ffx_interval &b) { return a.first < b.first; });
             Spans spans;
