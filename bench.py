@@ -233,14 +233,17 @@ def main():
     sp.close()
     qp.close()
     cpu = None if args.no_cpu else cpu_baseline(sum_limit, args.cpu_timeout)
-    batched = served = None
+    batched = served = served_warm = None
     if not args.no_modes:
         # 4096 seeded ranges (starts below 2^22, widths 2^8..2^16) in one
         # graph-captured persistent launch; the served Poisson stream of
         # 20000 requests (sumLimit up to 2^23, widths 2^6..2^18, 2000/s)
-        # with 100 of them replayed as sequential plain launches
+        # with 100 of them replayed as sequential plain launches; the map is
+        # grown on demand while requests arrive, and, in served_warm,
+        # sieved whole before the first arrival (as the plain launches have it)
         batched = mode_leg(["--batch", "--opt", "reps=3"], 300)
         served = mode_leg(["--serve", "--opt", "plain_sample=100"], 300)
+        served_warm = mode_leg(["--serve", "--opt", "warm=1"], 300)
     res = {
         "metric": METRIC,
         "value": nodd / (ms_per_step * 1e-3),
@@ -269,6 +272,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batched,
         "served": served,
+        "served_warm": served_warm,
     }
     if cpu and cpu.get("value"):
         res["gpu_over_cpu"] = round(res["value"] / cpu["value"], 1)

@@ -108,6 +108,16 @@ struct SearchArgs {
     uint64_t slotCap;         // bySum: slots outside [0, slotCap) are not written
 };
 
+// What the AllButOne check reads: the map and the smallest-factor table.  It
+// is passed by value to the out-of-line check, so the kernel argument block
+// itself never needs an address (an address would copy it to scratch).
+struct TermTables {
+    const uint8_t *map;
+    uint64_t M;
+    const uint16_t *spf;
+    uint32_t spfLimit;
+};
+
 // Sum of slot g: one range, or the interval holding g (binary search; the
 // wave's lanes agree on g, so the loads are uniform).
 __device__ __forceinline__ uint32_t sum_of_slot(const SearchArgs &a, uint32_t g)
@@ -350,7 +360,7 @@ __device__ __forceinline__ void probe_residues(const uint32_t *tab, uint32_t d, 
 // (LDS, this lane's row) and the 5-bit exponent fields of ex.  Quotients by
 // the factor p are exact, so they are products with p^-1 mod 2^32, and
 // p | n  <=>  n * p^-1 mod 2^32 <= (2^32-1)/p.
-__device__ __forceinline__ void factor_into(const SearchArgs &a, uint32_t n, uint32_t *pr, uint64_t &ex, int &nf)
+__device__ __forceinline__ void factor_into(const TermTables &a, uint32_t n, uint32_t *pr, uint64_t &ex, int &nf)
 {
     while (n > 1) {
         uint32_t p;
@@ -394,7 +404,7 @@ __device__ __forceinline__ void factor_into(const SearchArgs &a, uint32_t n, uin
 // is an exact double division (q < 2^53).  The f = 2^k pair is PTP(2^k + q),
 // already seen true by the caller for d > 1, and is the own pair for d = 1.
 // Terms beyond the map wait for Miller-Rabin until the map terms are done.
-__device__ __noinline__ bool ab1(const SearchArgs &a, uint32_t k, uint32_t d, uint32_t o, uint32_t *pr)
+__device__ __noinline__ bool ab1(const TermTables a, uint32_t k, uint32_t d, uint32_t o, uint32_t *pr)
 {
     const uint64_t q = (uint64_t)d * o;
     const uint64_t own = ((uint64_t)d << k) + o - 2;  // s - 2
@@ -473,7 +483,7 @@ __device__ __forceinline__ void drain(const SearchArgs &a, const uint2 *queue, i
         const uint32_t d = ent.x & ((1u << kDBits) - 1), k = ent.x >> kDBits;
         oo = ent.y;
         ee = d << k;
-        valid = ab1(a, k, d, oo, pr);
+        valid = ab1(TermTables{a.map, a.M, a.spf, a.spfLimit}, k, d, oo, pr);
     }
     const uint64_t vm = __ballot(valid);
     if (vm) {

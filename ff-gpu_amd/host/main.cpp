@@ -267,6 +267,7 @@ bool modeOptions(const Config &cfg, BatchOptions *b, ServeOptions *sv, AutotuneO
         else if (k == "batch_sums") sv->batchSums = u;
         else if (k == "streams") sv->streams = (uint32_t)u;
         else if (k == "plain_sample") sv->plainSample = (uint32_t)u;
+        else if (k == "warm") sv->warm = u != 0;
         else if (k == "json") sv->jsonOut = v;
         else if (k == "sum_limit") at->sumLimit = (uint32_t)u;
         else if (k == "out") at->out = v;

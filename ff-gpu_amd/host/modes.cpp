@@ -428,9 +428,11 @@ int runServe(const Config &cfg, const ServeOptions &opt)
     // a search over a partial map would decide the missing probe values by
     // Miller-Rabin, several times slower than waiting for the sieve.
     const uint64_t growStep = 16ull << 30;
-    uint64_t mapBytesDone = std::min<uint64_t>(mapBytesMax, 1ull << 30), growths = 1;
+    const auto tm0 = Clock::now();
+    uint64_t mapBytesDone = opt.warm ? mapBytesMax : std::min<uint64_t>(mapBytesMax, 1ull << 30), growths = 1;
     if ((rc = ffx_sieve_map_range_async(sp, dMap, 0, mapBytesDone, gst)) || (rc = ffx_stream_sync(gst)))
         return fail("map", rc);
+    const double mapBuildS = std::chrono::duration<double>(Clock::now() - tm0).count();
     uint64_t wantBytes = mapBytesDone, issuedBytes = mapBytesDone;
     std::deque<std::pair<void *, uint64_t>> growQ;  // (event, map bytes done when it fires)
     std::vector<void *> growEvents;
@@ -665,12 +667,13 @@ int runServe(const Config &cfg, const ServeOptions &opt)
                   "\"makespan_s\": %.3f, \"served_sums_per_s\": %.1f, \"searched_sums_per_s\": %.1f, "
                   "\"p50_ms\": %.3f, \"p99_ms\": %.3f, \"map_value\": %llu, \"map_bytes\": %llu, "
                   "\"map_growths\": %llu, \"solution_lines\": %llu, \"plain_sample\": %u, \"plain_span_s\": %.3f, "
-                  "\"plain_sums_per_s\": %.1f, \"plain_p50_ms\": %.3f, \"plain_p99_ms\": %.3f}\n",
+                  "\"plain_sums_per_s\": %.1f, \"plain_p50_ms\": %.3f, \"plain_p99_ms\": %.3f, \"warm\": %s, "
+                  "\"map_prebuilt_s\": %.3f}\n",
                   n, opt.rate, (unsigned long long)requested, (unsigned long long)searchedSlots,
                   (unsigned long long)batches, (unsigned long long)batchSums, S, makespan,
                   (double)requested / makespan, (double)searchedSlots / makespan, p50, p99, (unsigned long long)M,
                   (unsigned long long)mapBytesMax, (unsigned long long)growths, (unsigned long long)lines,
-                  opt.plainSample, plainSpan, plainRate, pp50, pp99);
+                  opt.plainSample, plainSpan, plainRate, pp50, pp99, opt.warm ? "true" : "false", mapBuildS);
     std::fputs(js, stdout);
     if (!opt.jsonOut.empty()) writeFile(opt.jsonOut, js);
     ffx_free_pinned(mirror);

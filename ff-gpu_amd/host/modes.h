@@ -29,6 +29,7 @@ struct ServeOptions {
     uint64_t batchSums = 65536;   // odd sums per persistent launch (the oldest queued first)
     uint32_t streams = 2;        // batches in flight
     uint32_t plainSample = 0;    // also time this many requests as sequential plain launches
+    bool warm = false;           // sieve the whole map before the first arrival (else grown on demand)
     std::string outDir;
     std::string jsonOut;         // summary line
 };
