@@ -103,8 +103,26 @@ __device__ __forceinline__ uint32_t to_map_order(uint32_t v)
     return __builtin_bswap32(__builtin_bitreverse32(v));
 }
 
-template <uint32_t kSegBytes>
-__global__ __launch_bounds__(kThreads) void sieve_segments(SieveArgs a)
+// Lane primes carried across a run: thread t keeps, for k < kCarry, the
+// offset of the next strike of primes[ncoop + t + k*kThreads] in a register,
+// so only the run's first segment divides; the primes beyond them start
+// each segment with first_strike.  kHoldPrimes keeps the primes themselves
+// in registers too; otherwise they are re-read each segment (L2-resident),
+// which halves the registers per carried prime.  32 KiB segments run eight
+// waves per SIMD (64 registers per lane), 64 KiB segments four (128).
+// Measured on MI355X (4 GiB / 16 GiB / 70 GB maps): 32 KiB with 12 re-read
+// carries 11.4 ms (8 held: 11.8); 64 KiB with 32 held 53.9 ms on 16 GiB
+// (64 re-read: 55.7) but 278 ms on 70 GB, where 64 re-read take 263.
+#ifndef FFX_FILL_UNROLL
+#define FFX_FILL_UNROLL 4
+#endif
+constexpr int kCarrySmall = 12;      // 32 KiB segments, primes re-read
+constexpr int kCarryHeld = 32;       // 64 KiB segments, primes held
+constexpr int kCarryReread = 64;     // 64 KiB segments, primes re-read
+constexpr uint32_t kHeldUpTo = 48 * 1024;  // lane primes up to which 64 KiB holds them
+
+template <uint32_t kSegBytes, int kCarry, bool kHoldPrimes>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kSegBytes == 32768 ? 8 : 4))) void sieve_segments(SieveArgs a)
 {
     constexpr uint32_t kSegWords = kSegBytes / 4;
     constexpr uint32_t kSegOdds = kSegBytes * 8;  // odd values per segment
@@ -142,6 +160,26 @@ __global__ __launch_bounds__(kThreads) void sieve_segments(SieveArgs a)
             stepSeg[g] = (uint32_t)(segStride % period[g]);
         }
     }
+    // carried lane primes: next strike relative to the current segment's
+    // start, at most the run's length (the launch keeps runs below 2^31 odd
+    // values), so a prime whose p*p lies beyond the run never strikes in it
+    uint32_t cn[kCarry], cp[kHoldPrimes ? kCarry : 1];
+    if (runLo < runHi) {
+        const uint64_t j0 = (a.segFirst + runLo) * (uint64_t)kSegOdds;
+        const uint64_t runOdds = (runHi - runLo) * (uint64_t)kSegOdds;
+#pragma unroll
+        for (int k = 0; k < kCarry; ++k) {
+            const uint32_t pi = a.ncoop + (uint32_t)tid + (uint32_t)k * kThreads;
+            cn[k] = 0;
+            if (kHoldPrimes) cp[kHoldPrimes ? k : 0] = 0;
+            if (pi < a.nprimes) {
+                const uint32_t p = a.primes[pi];
+                const uint64_t f = first_strike(j0, p, a.invp[pi]);
+                cn[k] = (uint32_t)(f < runOdds ? f : runOdds);
+                if (kHoldPrimes) cp[kHoldPrimes ? k : 0] = p;
+            }
+        }
+    }
     __syncthreads();
 
     for (uint64_t si = runLo; si < runHi; ++si) {
@@ -154,7 +192,7 @@ __global__ __launch_bounds__(kThreads) void sieve_segments(SieveArgs a)
             uint32_t c[kGroups];
 #pragma unroll
             for (int g = 0; g < kGroups; ++g) c[g] = cur[g];
-#pragma unroll 4
+#pragma unroll FFX_FILL_UNROLL
             for (uint32_t w = tid; w < kSegWords; w += kThreads) {
                 uint32_t v = 0;
 #pragma unroll
@@ -183,9 +221,18 @@ __global__ __launch_bounds__(kThreads) void sieve_segments(SieveArgs a)
                 } else {
                     first = coopNext[pi];
                 }
-                const uint32_t stride = 64u * p;
-                for (uint32_t j = first + (uint32_t)lane * p; j < limit; j += stride)
-                    atomicOr(&seg[j >> 5], 1u << (j & 31));
+                // lane l strikes multiples first + (l + 64t)p: a stride of 64p
+                // odd values is 2p whole words, so the bit stays put and only
+                // the word address advances (bits past limit in its last
+                // word are cleared again at write-out)
+                const uint32_t j = first + (uint32_t)lane * p;
+                if (j < limit) {
+                    const uint32_t mask = 1u << (j & 31);
+                    const uint32_t stepBytes = 8u * p, endBytes = ((limit + 31) >> 5) << 2;
+                    char *segBytes = reinterpret_cast<char *>(seg);
+                    for (uint32_t ad = (j >> 5) << 2; ad < endBytes; ad += stepBytes)
+                        atomicOr(reinterpret_cast<uint32_t *>(segBytes + ad), mask);
+                }
                 // first strike from the next segment's start: the first
                 // multiple at or after kSegOdds, n = ceil((kSegOdds-first)/p)
                 uint32_t next;
@@ -200,7 +247,23 @@ __global__ __launch_bounds__(kThreads) void sieve_segments(SieveArgs a)
                 }
                 if (lane == 0) coopNext[pi] = next;
             }
-            for (uint32_t pi = a.ncoop + tid; pi < a.nprimes; pi += kThreads) {
+            // after the last strike j >= limit; with limit == kSegOdds it is
+            // the next segment's first strike (a segment cut short by
+            // oddLimit is the last one the run strikes)
+            // re-read primes: the empty asm keeps the compiler from hoisting
+            // the loads out of the segment loop (holding them after all)
+            const uint32_t *cpr = a.primes + a.ncoop + tid;
+            if (!kHoldPrimes) asm volatile("" : "+v"(cpr));
+#pragma unroll
+            for (int k = 0; k < kCarry; ++k) {
+                if (a.ncoop + (uint32_t)tid + (uint32_t)k * kThreads < a.nprimes) {
+                    const uint32_t p = kHoldPrimes ? cp[kHoldPrimes ? k : 0] : cpr[k * kThreads];
+                    uint32_t j = cn[k];
+                    for (; j < limit; j += p) atomicOr(&seg[j >> 5], 1u << (j & 31));
+                    cn[k] = j - kSegOdds;
+                }
+            }
+            for (uint32_t pi = a.ncoop + (uint32_t)kCarry * kThreads + tid; pi < a.nprimes; pi += kThreads) {
                 const uint32_t p = a.primes[pi];
                 const uint64_t first = first_strike(j0, p, a.invp[pi]);
                 if (first >= limit) continue;
@@ -355,11 +418,20 @@ static int launch(const ffx_sieve_plan *pl, uint8_t *dst, uint64_t dstBase, uint
     a.segCount = (byteHi + segBytes - 1) / segBytes - a.segFirst;
     if (a.segFirst * segOdds >= (1ull << 52)) return fail(FFX_ERANGE, "sieve: map offset beyond 2^53 values");
     const uint64_t maxGrid = (uint64_t)pl->cus * pl->blocksPerCU;
-    const uint32_t grid = a.segCount < maxGrid ? (uint32_t)a.segCount : (uint32_t)maxGrid;
+    uint64_t grid = a.segCount < maxGrid ? a.segCount : maxGrid;
+    // each workgroup's run of segments stays below 2^31 odd values (its
+    // carried strike offsets are 32-bit)
+    const uint64_t minGrid = (a.segCount * segOdds + (1ull << 31) - 1) >> 31;
+    if (grid < minGrid) grid = minGrid;
     if (segBytes == 32768)
-        hipLaunchKernelGGL(sieve_segments<32768>, dim3(grid), dim3(kThreads), 0, as_stream(stream), a);
+        hipLaunchKernelGGL((sieve_segments<32768, kCarrySmall, false>), dim3((uint32_t)grid), dim3(kThreads), 0,
+                           as_stream(stream), a);
+    else if (a.nprimes - a.ncoop <= kHeldUpTo)
+        hipLaunchKernelGGL((sieve_segments<65536, kCarryHeld, true>), dim3((uint32_t)grid), dim3(kThreads), 0,
+                           as_stream(stream), a);
     else
-        hipLaunchKernelGGL(sieve_segments<65536>, dim3(grid), dim3(kThreads), 0, as_stream(stream), a);
+        hipLaunchKernelGGL((sieve_segments<65536, kCarryReread, false>), dim3((uint32_t)grid), dim3(kThreads), 0,
+                           as_stream(stream), a);
     FFX_HIP(hipGetLastError());
     return FFX_OK;
 }
