@@ -75,6 +75,7 @@ struct SieveArgs {
     const double *invp;      // 1.0 / primes[i]
     uint32_t nprimes;
     uint32_t ncoop;          // primes[0, ncoop) strike wave-cooperatively (> 64 strikes per segment)
+    uint32_t ncarry;         // lane primes carried per thread (<= the kernel's kCarry)
     const uint32_t *patterns;  // kPatWords words, groups back to back
     uint64_t segFirst, segCount;
 };
@@ -172,7 +173,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kSegBy
             const uint32_t pi = a.ncoop + (uint32_t)tid + (uint32_t)k * kThreads;
             cn[k] = 0;
             if (kHoldPrimes) cp[kHoldPrimes ? k : 0] = 0;
-            if (pi < a.nprimes) {
+            if ((uint32_t)k < a.ncarry && pi < a.nprimes) {
                 const uint32_t p = a.primes[pi];
                 const uint64_t f = first_strike(j0, p, a.invp[pi]);
                 cn[k] = (uint32_t)(f < runOdds ? f : runOdds);
@@ -256,14 +257,14 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kSegBy
             if (!kHoldPrimes) asm volatile("" : "+v"(cpr));
 #pragma unroll
             for (int k = 0; k < kCarry; ++k) {
-                if (a.ncoop + (uint32_t)tid + (uint32_t)k * kThreads < a.nprimes) {
+                if ((uint32_t)k < a.ncarry && a.ncoop + (uint32_t)tid + (uint32_t)k * kThreads < a.nprimes) {
                     const uint32_t p = kHoldPrimes ? cp[kHoldPrimes ? k : 0] : cpr[k * kThreads];
                     uint32_t j = cn[k];
                     for (; j < limit; j += p) atomicOr(&seg[j >> 5], 1u << (j & 31));
                     cn[k] = j - kSegOdds;
                 }
             }
-            for (uint32_t pi = a.ncoop + (uint32_t)kCarry * kThreads + tid; pi < a.nprimes; pi += kThreads) {
+            for (uint32_t pi = a.ncoop + a.ncarry * kThreads + tid; pi < a.nprimes; pi += kThreads) {
                 const uint32_t p = a.primes[pi];
                 const uint64_t first = first_strike(j0, p, a.invp[pi]);
                 if (first >= limit) continue;
@@ -334,6 +335,8 @@ struct ffx_sieve_plan {
     uint32_t nprimes;
     std::vector<uint32_t> hostPrimes;  // the struck primes (> 61)
     uint32_t segBytes, blocksPerCU, cus;
+    uint32_t laneCarry;   // at most this many carried lane primes per thread (FFX_SIEVE_LANE_CARRY)
+    uint32_t holdPrimes;  // 64 KiB segments: 0 by the lane-prime count, 1 held, 2 re-read
     uint32_t *dPrimes;
     double *dInv;
     uint32_t *dPatterns;
@@ -374,6 +377,8 @@ static int plan_build(ffx_sieve_plan **out, std::vector<uint32_t> &oddPrimes, ui
     const bool large = M >= kLargeMapFrom;
     pl->segBytes = large ? kLargeMapSegBytes : kSmallMapSegBytes;
     pl->blocksPerCU = large ? kLargeMapBlocksPerCU : kSmallMapBlocksPerCU;
+    pl->laneCarry = kCarryReread;
+    pl->holdPrimes = 0;
     hipError_t e = hipGetDevice(&pl->device);
     int cus = 0;
     if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, pl->device);
@@ -423,15 +428,20 @@ static int launch(const ffx_sieve_plan *pl, uint8_t *dst, uint64_t dstBase, uint
     // carried strike offsets are 32-bit)
     const uint64_t minGrid = (a.segCount * segOdds + (1ull << 31) - 1) >> 31;
     if (grid < minGrid) grid = minGrid;
-    if (segBytes == 32768)
+    const bool hold = pl->holdPrimes == 1 || (pl->holdPrimes == 0 && a.nprimes - a.ncoop <= kHeldUpTo);
+    if (segBytes == 32768) {
+        a.ncarry = std::min<uint32_t>(pl->laneCarry, kCarrySmall);
         hipLaunchKernelGGL((sieve_segments<32768, kCarrySmall, false>), dim3((uint32_t)grid), dim3(kThreads), 0,
                            as_stream(stream), a);
-    else if (a.nprimes - a.ncoop <= kHeldUpTo)
+    } else if (hold) {
+        a.ncarry = std::min<uint32_t>(pl->laneCarry, kCarryHeld);
         hipLaunchKernelGGL((sieve_segments<65536, kCarryHeld, true>), dim3((uint32_t)grid), dim3(kThreads), 0,
                            as_stream(stream), a);
-    else
+    } else {
+        a.ncarry = std::min<uint32_t>(pl->laneCarry, kCarryReread);
         hipLaunchKernelGGL((sieve_segments<65536, kCarryReread, false>), dim3((uint32_t)grid), dim3(kThreads), 0,
                            as_stream(stream), a);
+    }
     FFX_HIP(hipGetLastError());
     return FFX_OK;
 }
@@ -494,6 +504,14 @@ int ffx_sieve_plan_set(ffx_sieve_plan *plan, int param, uint32_t value)
         if (value < 1 || value > 8) return fail(FFX_ERANGE, "sieve blocks per CU %u outside 1..8", value);
         plan->blocksPerCU = value;
         return FFX_OK;
+    case FFX_SIEVE_LANE_CARRY:
+        if (value > (uint32_t)kCarryReread) return fail(FFX_ERANGE, "sieve lane carry %u outside 0..%d", value, kCarryReread);
+        plan->laneCarry = value;
+        return FFX_OK;
+    case FFX_SIEVE_HOLD_PRIMES:
+        if (value > 2) return fail(FFX_ERANGE, "sieve hold primes %u outside 0..2", value);
+        plan->holdPrimes = value;
+        return FFX_OK;
     default:
         return fail(FFX_EINVAL, "sieve plan: unknown parameter %d", param);
     }
@@ -505,6 +523,8 @@ int ffx_sieve_plan_get(const ffx_sieve_plan *plan, int param, uint32_t *value)
     switch (param) {
     case FFX_SIEVE_SEGMENT_BYTES: *value = plan->segBytes; return FFX_OK;
     case FFX_SIEVE_BLOCKS_PER_CU: *value = plan->blocksPerCU; return FFX_OK;
+    case FFX_SIEVE_LANE_CARRY: *value = plan->laneCarry; return FFX_OK;
+    case FFX_SIEVE_HOLD_PRIMES: *value = plan->holdPrimes; return FFX_OK;
     default: return fail(FFX_EINVAL, "sieve plan: unknown parameter %d", param);
     }
 }

@@ -100,7 +100,7 @@ _SIGS = {
 }
 
 SEARCH_GRAB, SEARCH_EAGER_SPLITS, SEARCH_BLOCKS_PER_CU = 1, 2, 3
-SIEVE_SEGMENT_BYTES, SIEVE_BLOCKS_PER_CU = 1, 2
+SIEVE_SEGMENT_BYTES, SIEVE_BLOCKS_PER_CU, SIEVE_LANE_CARRY, SIEVE_HOLD_PRIMES = 1, 2, 3, 4
 
 _lib = None
 

@@ -79,6 +79,34 @@ def test_map_grown_in_ranges_equals_full_map():
     assert np.array_equal(buf.to_host(len(ref)), ref)
 
 
+# (segment bytes, hold primes, lane carry): every kernel variant, with its
+# carried lane primes covering none, part or all of the 3,500 sieving primes
+# of a 1 GiB map (1024 threads per workgroup), so both the carried strikes
+# and the per-segment first multiples run
+CARRY_CASES = [(32768, 0, 0), (32768, 0, 1), (32768, 0, 64), (65536, 1, 0), (65536, 1, 2), (65536, 1, 64),
+               (65536, 2, 0), (65536, 2, 3), (65536, 2, 64)]
+
+
+@pytest.mark.parametrize("seg,hold,carry", CARRY_CASES)
+def test_lane_carry_variants_match_oracle(seg, hold, carry):
+    ref = oracle_map(131071)
+    M = len(ref) * 16 - 1
+    say("carry variant", seg, hold, carry, M)
+    plan = ffxpy.SievePlan(M)
+    lib = ffxpy.load()
+    for param, v in ((ffxpy.SIEVE_SEGMENT_BYTES, seg), (ffxpy.SIEVE_HOLD_PRIMES, hold), (ffxpy.SIEVE_LANE_CARRY, carry)):
+        ffxpy.check(lib.ffx_sieve_plan_set(plan.h, param, v))
+    buf = ffxpy.sieve_map(M, plan)
+    assert np.array_equal(buf.to_host(len(ref)), ref)
+    # ranges grown on the same plan (runs that start mid-map)
+    buf2 = ffxpy.DeviceBuffer((len(ref) + 255) & ~255)
+    cuts = [0, 3 * 65536 + 5, 10 ** 7 + 1, len(ref)]
+    for a, b in zip(cuts, cuts[1:]):
+        ffxpy.check(lib.ffx_sieve_map_range_async(plan.h, buf2.ptr, a, b, None))
+    ffxpy.check(lib.ffx_device_sync())
+    assert np.array_equal(buf2.to_host(len(ref)), ref)
+
+
 def test_plan_from_caller_primes():
     ref = oracle_map(65535)
     M = len(ref) * 16 - 1
