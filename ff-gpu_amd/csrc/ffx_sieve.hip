@@ -10,17 +10,18 @@
 //
 // Kernel design (gfx950, wave64, 160 KiB LDS per CU):
 //  * One workgroup of 1024 threads owns one map segment at a time in LDS
-//    (32 KiB = 2^18 odd values by default, 64 KiB selectable) and walks
-//    segments in a grid-stride loop.
+//    (32 KiB = 2^18 odd values below maps of 2^38 values, 64 KiB from there)
+//    and sieves a contiguous run of segments.
 //  * Bits are kept "struck" = 1 in LDS, in natural order (bit b of word w is
 //    odd index 32w+b); the write-out inverts them and reorders each byte to
 //    the reference's MSB-first layout (bit position b^7 within the word).
 //  * Primes 3..61 never strike: their multiples are OR-ed in as periodic word
 //    patterns (eight groups, periods 105..3599 words) kept in LDS.
-//  * Primes 67..~8191 (more than 64 strikes per segment) are struck
-//    cooperatively: one wavefront per prime, lane l striking every 64th
-//    multiple.  Larger primes are struck one prime per lane.  Strikes are
-//    LDS atomic ORs (ds_or_b32).
+//  * Primes 67..segOdds/64 (more than 64 strikes per segment) are struck
+//    cooperatively: two primes per wavefront, lane l of a half striking
+//    every 32nd multiple (a fixed bit, an advancing word).  Larger primes
+//    are struck one prime per lane, carrying their next strike across the
+//    workgroup's run of segments.  Strikes are LDS atomic ORs (ds_or_b32).
 //  * The segment leaves LDS through 16-byte coalesced stores.
 #include <algorithm>
 #include <cmath>
@@ -211,42 +212,47 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kSegBy
         const uint64_t limit64 = a.oddLimit > j0 ? a.oddLimit - j0 : 0;
         const uint32_t limit = limit64 < kSegOdds ? (uint32_t)limit64 : kSegOdds;
         if (limit) {
-            // the same wave owns prime pi in every segment of the run, so
-            // coopNext[pi] is written and read by one wave only
-            for (uint32_t pi = wave; pi < a.ncoop; pi += kWaves) {
-                const uint32_t p = a.primes[pi];
-                uint32_t first;
-                if (si == runLo) {
-                    const uint64_t f = first_strike(j0, p, a.invp[pi]);  // < (p*p-1)/2 < 2^32
-                    first = (uint32_t)f;
-                } else {
-                    first = coopNext[pi];
+            // two primes per wavefront, one per half: the same half owns
+            // prime pi in every segment of the run, so coopNext[pi] is
+            // written and read by one half-wave only
+            const uint32_t half = (uint32_t)lane >> 5, hl = (uint32_t)lane & 31;
+            for (uint32_t pb = 2 * (uint32_t)wave; pb < a.ncoop; pb += 2 * kWaves) {
+                const uint32_t pi = pb + half;
+                if (pi < a.ncoop) {
+                    const uint32_t p = a.primes[pi];
+                    uint32_t first;
+                    if (si == runLo) {
+                        const uint64_t f = first_strike(j0, p, a.invp[pi]);  // < (p*p-1)/2 < 2^32
+                        first = (uint32_t)f;
+                    } else {
+                        first = coopNext[pi];
+                    }
+                    // lane l of the half strikes multiples first + (l + 32t)p:
+                    // a stride of 32p odd values is p whole words, so the bit
+                    // stays put and only the word address advances (bits past
+                    // limit in its last word are cleared again at write-out)
+                    const uint32_t j = first + __umul24(hl, p);  // p < 2^16
+                    if (j < limit) {
+                        const uint32_t mask = 1u << (j & 31);
+                        const uint32_t stepBytes = 4u * p, endBytes = ((limit + 31) >> 5) << 2;
+                        char *segBytes = reinterpret_cast<char *>(seg);
+                        for (uint32_t ad = (j >> 5) << 2; ad < endBytes; ad += stepBytes)
+                            atomicOr(reinterpret_cast<uint32_t *>(segBytes + ad), mask);
+                    }
+                    // first strike from the next segment's start: the first
+                    // multiple at or after kSegOdds, n = ceil((kSegOdds-first)/p)
+                    uint32_t next;
+                    if (first >= kSegOdds) {
+                        next = first - kSegOdds;
+                    } else {
+                        const uint32_t D = kSegOdds - first;  // < 2^20: the float quotient is off by at most one
+                        uint32_t q = (uint32_t)((float)D * __builtin_amdgcn_rcpf((float)p));
+                        if (q * p > D) --q;
+                        if ((q + 1) * p <= D) ++q;
+                        next = first + (q + (q * p < D ? 1u : 0u)) * p - kSegOdds;
+                    }
+                    if (hl == 0) coopNext[pi] = next;
                 }
-                // lane l strikes multiples first + (l + 64t)p: a stride of 64p
-                // odd values is 2p whole words, so the bit stays put and only
-                // the word address advances (bits past limit in its last
-                // word are cleared again at write-out)
-                const uint32_t j = first + (uint32_t)lane * p;
-                if (j < limit) {
-                    const uint32_t mask = 1u << (j & 31);
-                    const uint32_t stepBytes = 8u * p, endBytes = ((limit + 31) >> 5) << 2;
-                    char *segBytes = reinterpret_cast<char *>(seg);
-                    for (uint32_t ad = (j >> 5) << 2; ad < endBytes; ad += stepBytes)
-                        atomicOr(reinterpret_cast<uint32_t *>(segBytes + ad), mask);
-                }
-                // first strike from the next segment's start: the first
-                // multiple at or after kSegOdds, n = ceil((kSegOdds-first)/p)
-                uint32_t next;
-                if (first >= kSegOdds) {
-                    next = first - kSegOdds;
-                } else {
-                    const uint32_t D = kSegOdds - first;  // < 2^20, so the float quotient is off by at most one
-                    uint32_t q = (uint32_t)((float)D * __builtin_amdgcn_rcpf((float)p));
-                    if (q * p > D) --q;
-                    if ((q + 1) * p <= D) ++q;
-                    next = first + (q + (q * p < D ? 1u : 0u)) * p - kSegOdds;
-                }
-                if (lane == 0) coopNext[pi] = next;
             }
             // after the last strike j >= limit; with limit == kSegOdds it is
             // the next segment's first strike (a segment cut short by
