@@ -56,6 +56,59 @@ def traffic_for(kernel, workload):
     return t
 
 
+def probes_for(workload):
+    """Map reads of one search launch, by kind (profiles/search_probes.json,
+    written by `bench.py --count-probes` with the counters build)."""
+    p = os.path.join(ROOT, "profiles", "search_probes.json")
+    if not os.path.exists(p):
+        return None
+    return json.load(open(p)).get(workload)
+
+
+def random_read_ceiling():
+    """The device's rate of random single-byte reads over a 4 GiB buffer
+    (bin/ff_random_reads), the access pattern of the search's map probes."""
+    exe = os.path.join(ROOT, "ff-gpu_amd", "bin", "ff_random_reads")
+    if not os.path.exists(exe):
+        return None
+    try:
+        p = subprocess.run([exe], capture_output=True, timeout=120)
+    except subprocess.TimeoutExpired:
+        return None
+    if p.returncode != 0:
+        return None
+    return json.loads(p.stdout.decode().strip().splitlines()[-1])
+
+
+COUNTER_NAMES = ["sums_enumerated", "splits_visited", "splits_past_residue_filter", "reads_2k_plus_do",
+                 "reads_d_plus_o2k", "reads_o", "allbutone_checks", "reads_in_allbutone", "miller_rabin_tests",
+                 "splits_listed"]
+
+
+def count_probes(workload):
+    """One search of the workload with lib/libffx_counters.so: the counts of
+    map reads and the other work by kind (the kernel's own, not the
+    reference's order)."""
+    import numpy as np
+    import ffxpy
+    ffxpy.LIB_PATH = os.path.join(ROOT, "ff-gpu_amd", "lib", "libffx_counters.so")
+    lib = ffxpy.load()
+    lib.ffx_probe_counters.restype = C.c_int
+    lib.ffx_probe_counters.argtypes = [C.c_void_p, C.c_int]
+    ffxpy.init(0)
+    sum_limit = WORKLOADS[workload][0]
+    M = 68719476751 if workload == "sums2p20" else 274877906959
+    dmap = ffxpy.sieve_map(M)
+    arr = np.zeros(16, dtype=np.uint64)
+    ffxpy.check(lib.ffx_probe_counters(arr.ctypes.data, 1))
+    sol = ffxpy.search(dmap, M, 5, sum_limit)
+    ffxpy.check(lib.ffx_probe_counters(arr.ctypes.data, 1))
+    res = {n: int(arr[i]) for i, n in enumerate(COUNTER_NAMES)}
+    res["map_reads"] = res["reads_2k_plus_do"] + res["reads_d_plus_o2k"] + res["reads_o"] + res["reads_in_allbutone"]
+    res["solutions"] = int(len(sol))
+    print(json.dumps({workload: res}))
+
+
 def cpu_baseline(sum_limit, timeout_s):
     """The reference program (oracle/_ref/ff_seg, built from the reference's
     source, 31 threads) on the same workload, timed by its own clocks."""
@@ -129,11 +182,16 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-timeout", type=float, default=150.0)
     ap.add_argument("--no-modes", action="store_true", help="skip the batched and served legs")
+    ap.add_argument("--count-probes", action="store_true",
+                    help="only count the search's map reads by kind (needs make -C ff-gpu_amd counters)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only copy a BYTES device buffer three times: the run that scales the HBM byte counters")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
+        return
+    if args.count_probes:
+        count_probes(args.workload)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the sieve and search are single-device here (each GPU would "
@@ -232,6 +290,17 @@ def main():
         b.close()
     sp.close()
     qp.close()
+    # the search's map reads (counted) against the device's measured rate of
+    # random single-byte reads: the roof for this access pattern, where every
+    # read costs a whole 128-byte line
+    ceiling = random_read_ceiling()
+    pc = probes_for(args.workload)
+    if ceiling and pc:
+        rate = pc["map_reads"] / (search_avg * 1e-3)
+        roof["random_read_roof"] = {"map_reads_per_launch": pc["map_reads"], "achieved_reads_per_s": round(rate, -6),
+                                    "ceiling_reads_per_s": round(ceiling["reads_per_s"], -6),
+                                    "frac": round(rate / ceiling["reads_per_s"], 3),
+                                    "ceiling_from": "bin/ff_random_reads (4 GiB buffer, 2^28 reads, best of 3)"}
     cpu = None if args.no_cpu else cpu_baseline(sum_limit, args.cpu_timeout)
     batched = served = served_warm = None
     if not args.no_modes:

@@ -44,6 +44,18 @@
 
 #include "ffx_internal.h"
 
+// Work counters, compiled in only for the diagnostic library
+// (make -C ff-gpu_amd counters -> lib/libffx_counters.so, FFX_PROBE_COUNTERS):
+// map reads by kind, AllButOne calls, Miller-Rabin tests and splits, summed
+// over a launch with device atomics (which slow the kernel; the counts, not
+// the time, are what that build is for).
+#ifdef FFX_PROBE_COUNTERS
+__device__ unsigned long long g_ffxCounters[16];
+#define FFX_COUNT(i, v) atomicAdd(&g_ffxCounters[i], (unsigned long long)(v))
+#else
+#define FFX_COUNT(i, v) ((void)0)
+#endif
+
 namespace {
 
 constexpr int kBlock = 256;
@@ -209,6 +221,7 @@ __device__ bool mr_is_prime_f64(uint64_t n)
 // form.
 __device__ __noinline__ bool mr_is_prime(uint64_t n)
 {
+    FFX_COUNT(8, 1);
     if (n < (1ull << 50)) return mr_is_prime_f64(n);
     uint64_t inv = n;  // n * n == 1 mod 8
 #pragma unroll
@@ -417,6 +430,7 @@ __device__ __noinline__ bool ab1(const TermTables a, uint32_t k, uint32_t d, uin
         const uint64_t x_ = (X);                        \
         if (x_ != own) {                                \
             if (x_ <= a.M) {                            \
+                FFX_COUNT(7, 1);                        \
                 if (!map_bit(a.map, x_)) return false;  \
             } else if (!late0) {                        \
                 late0 = x_;                             \
@@ -485,6 +499,9 @@ __device__ __forceinline__ void drain(const SearchArgs &a, const uint2 *queue, i
         ee = d << k;
         valid = ab1(TermTables{a.map, a.M, a.spf, a.spfLimit}, k, d, oo, pr);
     }
+#ifdef FFX_PROBE_COUNTERS
+    if (lane == 0) FFX_COUNT(6, nb);
+#endif
     const uint64_t vm = __ballot(valid);
     if (vm) {
         const int l = __ffsll((unsigned long long)vm) - 1;
@@ -523,6 +540,13 @@ __device__ __forceinline__ void probe_batch(const SearchArgs &a, const uint2 *pq
     for (int u = 0; u < 2; ++u) {
         oo[u] = (needO[u] && o[u] <= a.M) ? map_byte(a.map, o[u]) : 0u;
         ob[u] = (st[u] && x[u] <= a.M) ? map_byte(a.map, x[u]) : 0u;
+#ifdef FFX_PROBE_COUNTERS
+        const uint64_t b1 = __ballot(needO[u] && o[u] <= a.M), b2 = __ballot(st[u] && x[u] <= a.M);
+        if (lane == 0) {
+            FFX_COUNT(5, __popcll(b1));
+            FFX_COUNT(3, __popcll(b2));
+        }
+#endif
     }
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
@@ -536,6 +560,10 @@ __device__ __forceinline__ void probe_batch(const SearchArgs &a, const uint2 *pq
         need2[u] = st[u] && d[u] != o[u];
         x[u] = (uint64_t)d[u] + ((uint64_t)o[u] << k[u]) - 2;
         ob[u] = (need2[u] && x[u] <= a.M) ? map_byte(a.map, x[u]) : 0u;
+#ifdef FFX_PROBE_COUNTERS
+        const uint64_t b3 = __ballot(need2[u] && x[u] <= a.M);
+        if (lane == 0) FFX_COUNT(4, __popcll(b3));
+#endif
     }
 #pragma unroll
     for (int u = 0; u < 2; ++u)
@@ -651,7 +679,16 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
     // the sum is young, queues are emptied every round (early exit at two
     // valid splits).
     int hint = 0, pn = 0;
+#ifdef FFX_PROBE_COUNTERS
+    if (lane == 0) {
+        FFX_COUNT(0, 1);
+        FFX_COUNT(9, total);
+    }
+#endif
     for (uint32_t base = 0; base < total; base += 64 * kRoundSplits) {
+#ifdef FFX_PROBE_COUNTERS
+        if (lane == 0) FFX_COUNT(1, total - base < 64u * kRoundSplits ? total - base : 64u * kRoundSplits);
+#endif
         int segOf[kRoundSplits];
 #pragma unroll
         for (int u = 0; u < kRoundSplits; ++u) {
@@ -680,6 +717,9 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
             const bool keep = live && rx && ry && (!needO || ro);
             const uint64_t km = __ballot(keep);
             if (keep) pq[pn + __popcll(km & lt)] = make_uint2(d | (k << kDBits), o | (needO ? kNeedO : 0u));
+#ifdef FFX_PROBE_COUNTERS
+            if (lane == 0) FFX_COUNT(2, __popcll(km));
+#endif
             pn += __popcll(km);
         }
         hint = __shfl(segOf[kRoundSplits - 1], 63);
@@ -899,6 +939,19 @@ static int ensure_scratch(ffx_search_plan *pl, uint64_t nslots, bool slots)
 }
 
 extern "C" {
+
+#ifdef FFX_PROBE_COUNTERS
+int ffx_probe_counters(uint64_t *out, int reset)
+{
+    if (!out) return fail(FFX_EINVAL, "probe counters: null");
+    FFX_HIP(hipMemcpyFromSymbol(out, HIP_SYMBOL(g_ffxCounters), sizeof(uint64_t) * 16));
+    if (reset) {
+        const uint64_t z[16] = {};
+        FFX_HIP(hipMemcpyToSymbol(HIP_SYMBOL(g_ffxCounters), z, sizeof z));
+    }
+    return FFX_OK;
+}
+#endif
 
 int ffx_search_plan_create(ffx_search_plan **out, uint32_t sumLimitMax)
 {
