@@ -101,6 +101,35 @@ constexpr uint32_t tab_offset(int i) { return i == 0 ? 0 : tab_offset(i - 1) + (
 constexpr uint32_t kTabWords = tab_offset(kNMods);
 constexpr uint32_t kLargestResiduePrime = kModTopPrime[kNMods - 1];
 
+// Sieve over d: in a long k-segment, a chunk of kSieveBits consecutive odd d
+// is marked where 2^k + d*o - 2 (quadratic in d: two residue classes mod q
+// at most) or d + o*2^k - 2 (linear: one class) is divisible by a prime q
+// of kSievePrimes, one lane per (prime, value, root); splits on marked d are
+// dropped before any map read.  Both values exceed 2*sqrt(s-1) - 2 > 251
+// for every split of a sum s >= kSieveMinSum, so a marked value is
+// composite.  The chunk's bits live in 512 bytes of LDS per wave.
+constexpr uint32_t kSievePrimes[43] = {37,  41,  43,  47,  53,  59,  61,  67,  71,  73,  79,  83,  89,  97,  101,
+                                       103, 107, 109, 113, 127, 131, 137, 139, 149, 151, 157, 163, 167, 173, 179,
+                                       181, 191, 193, 197, 199, 211, 223, 227, 229, 233, 239, 241, 251};
+// Primes in use: 21 (37..131) measured as fast as 32 or 43 and faster than 14
+// or 8 (odd sums [5, 2^20): 16.5 / 16.6 / 16.5 / 16.9 / 17.1 ms).
+constexpr int kNSievePrimes = 21;  // (prime, value, root) items: 3 per prime
+constexpr uint32_t kSieveBits = 4096;            // odd d per chunk (512 bytes of LDS per wave)
+constexpr uint32_t kSieveMinSum = 1u << 15;
+constexpr uint32_t kSieveMinListed = 2048;  // segments shorter than this are not sieved
+constexpr uint32_t sieve_sqrt_offset(int i) { return i == 0 ? 0 : sieve_sqrt_offset(i - 1) + kSievePrimes[i - 1]; }
+constexpr uint32_t kSieveSqrtBytes = sieve_sqrt_offset(kNSievePrimes);
+static_assert(2 * 181 - 2 > 251, "sieve primes below the smallest probe value of a sum >= kSieveMinSum");
+static_assert(kNSievePrimes <= 43, "square roots and inverses are single bytes (primes below 255)");
+struct SieveSqrtOffsets {
+    uint32_t v[43];
+    constexpr SieveSqrtOffsets() : v()
+    {
+        for (int i = 0; i < 43; ++i) v[i] = sieve_sqrt_offset(i);
+    }
+};
+__constant__ constexpr SieveSqrtOffsets kSieveSqrtOff{};
+
 struct SearchArgs {
     const uint8_t *map;
     uint64_t M;               // bitmap answers odd n <= M
@@ -118,6 +147,8 @@ struct SearchArgs {
     uint32_t bySum;           // slot of sum s is slots[(s - sumBase) / 2] instead of slot g
     uint32_t sumBase;
     uint64_t slotCap;         // bySum: slots outside [0, slotCap) are not written
+    const uint32_t *sieveK;   // per sieve prime i and k < 32: 2^k mod q | (1-4^k)^-1 << 8 | (2^(k+1))^-1 << 16 | inverse-exists << 24
+    const uint8_t *sieveSqrt; // per sieve prime, q bytes: a square root of each residue, 0xFF for non-residues
 };
 
 // What the AllButOne check reads: the map and the smallest-factor table.  It
@@ -576,9 +607,50 @@ __device__ __forceinline__ void probe_batch(const SearchArgs &a, const uint2 *pq
     }
 }
 
+// Marks, in bits (kSieveBits, bit b = odd d = dLo + 2b), the d of k-segment
+// splits of sum s whose 2^k + d*o - 2 or d + o*2^k - 2 has a sieve prime
+// factor (o = s - d*2^k).  Lane 3i+r takes prime i: r = 0 the root of the
+// linear value, r = 1, 2 the roots (sm +- sqrt(D)) / 2^(k+1) of the
+// quadratic 2^k d^2 - s d - (2^k - 2) = 0 with D = s^2 + 4 * 2^k (2^k - 2).
+using lds_u32 = __attribute__((address_space(3))) uint32_t;
+__device__ __noinline__ void sieve_chunk(const uint32_t *sieveK, const uint8_t *sieveSqrt, uint32_t s, uint32_t k,
+                                        uint32_t dLo, lds_u32 *bits, int lane)
+{
+    for (uint32_t i = (uint32_t)lane; i < kSieveBits / 32; i += 64) bits[i] = 0;
+    wave_sync_lds();
+    for (int item = lane; item < 3 * kNSievePrimes; item += 64) {
+        const int pi = item / 3, which = item - 3 * pi;
+        const uint32_t q = kSievePrimes[pi];
+        const uint32_t rec = sieveK[pi * 32 + k];
+        const uint32_t pk = rec & 255, iY = (rec >> 8) & 255, iX = (rec >> 16) & 255;
+        const uint32_t sm = s % q;
+        int root = -1;
+        bool all = false;
+        if (which == 0) {
+            const uint32_t rhs = (2 + q - (sm * pk) % q) % q;  // d (1 - 4^k) = 2 - s 2^k
+            if (rec >> 24) root = (int)((rhs * iY) % q);
+            else all = rhs == 0;  // 4^k = 1 mod q: the value is s 2^k - 2 for every d
+        } else {
+            const uint32_t D = (sm * sm + 4 * pk * ((pk + q - 2) % q)) % q;
+            const uint32_t r = sieveSqrt[kSieveSqrtOff.v[pi] + D];
+            if (r != 0xFFu) root = (int)((((which == 1 ? sm + r : sm + q - r) % q) * iX) % q);
+        }
+        if (all) {
+            for (uint32_t i = 0; i < kSieveBits / 32; ++i)
+                __hip_atomic_fetch_or(&bits[i], 0xFFFFFFFFu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        } else if (root >= 0) {
+            uint32_t t = ((uint32_t)root + q - dLo % q) % q;  // first d >= dLo with d = root mod q
+            if (!((dLo + t) & 1)) t += q;                     // odd d (dLo is odd)
+            for (uint32_t b = t >> 1; b < kSieveBits; b += q)
+                __hip_atomic_fetch_or(&bits[b >> 5], 1u << (b & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        }
+    }
+    wave_sync_lds();
+}
+
 __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const uint32_t *tab, uint32_t s, int lane,
                                                    uint2 *queue, uint2 *pq, uint4 *segs, uint8_t *segOffs,
-                                                   uint32_t *pr)
+                                                   uint32_t *pr, uint32_t *dbits)
 {
     // statement 1 (RunIt :806-807)
     if (is_prime_odd(a, s - 2)) return 0;
@@ -685,7 +757,35 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
         FFX_COUNT(9, total);
     }
 #endif
+    // the chunk of the d sieve: segment sj, odd d in [sLo, sLo + 2 kSieveBits)
+    int sj = -1;
+    uint32_t sLo = 0;
+    const bool sieveSum = s >= kSieveMinSum;
     for (uint32_t base = 0; base < total; base += 64 * kRoundSplits) {
+        bool useSieve = false;
+        if (sieveSum && base >= a.eager) {
+            // uniform: the round's first and last split, their segment and d
+            const uint32_t gl = (base + 64 * kRoundSplits < total ? base + 64 * kRoundSplits : total) - 1;
+            int j0 = hint;
+            while (j0 + 1 < nseg && base >= segs[j0 + 1].x) ++j0;
+            const uint32_t segEnd = j0 + 1 < nseg ? segs[j0 + 1].x : total;
+            if (gl < segEnd && segEnd - segs[j0].x >= kSieveMinListed) {
+                const uint4 e = segs[j0];
+                const uint32_t c = (e.y >> 8) & 15;
+                auto d_at = [&](uint32_t g) {
+                    const uint32_t ii = g - e.x + (e.y >> 16);
+                    const uint32_t q = c == 1 ? ii : __umulhi(ii, e.w);
+                    return 30 * q + segOffs[j0 * 16 + (ii - q * c)];
+                };
+                const uint32_t d0 = d_at(base), d1 = d_at(gl);
+                if (j0 != sj || d0 < sLo || d1 >= sLo + 2 * kSieveBits) {
+                    sieve_chunk(a.sieveK, a.sieveSqrt, s, e.y & 31, d0, (lds_u32 *)dbits, lane);
+                    sj = j0;
+                    sLo = d0;
+                }
+                useSieve = true;
+            }
+        }
 #ifdef FFX_PROBE_COUNTERS
         if (lane == 0) FFX_COUNT(1, total - base < 64u * kRoundSplits ? total - base : 64u * kRoundSplits);
 #endif
@@ -714,7 +814,11 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
             }
             ry = ry || d == o;
             ro = ro || o <= kLargestResiduePrime;
-            const bool keep = live && rx && ry && (!needO || ro);
+            bool keep = live && rx && ry && (!needO || ro);
+            if (useSieve && keep && d != o) {
+                const uint32_t b = (d - sLo) >> 1;
+                keep = !((dbits[b >> 5] >> (b & 31)) & 1u);
+            }
             const uint64_t km = __ballot(keep);
             if (keep) pq[pn + __popcll(km & lt)] = make_uint2(d | (k << kDBits), o | (needO ? kNeedO : 0u));
 #ifdef FFX_PROBE_COUNTERS
@@ -778,6 +882,7 @@ __global__ __launch_bounds__(kBlock, FFX_SEARCH_OCCUPANCY) void search_sums(Sear
     __shared__ uint8_t segOffs[kWavesPerBlock][kMaxSegs * 16];  // allowed d mod 30 per segment
     __shared__ uint32_t primes[kBlock][kMaxPrimes];
     __shared__ uint32_t tab[kTabWords];
+    __shared__ uint32_t dsieve[kWavesPerBlock][kSieveBits / 32];
     for (uint32_t i = threadIdx.x; i < kTabWords; i += kBlock) tab[i] = a.coprime[i];
     __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -790,7 +895,8 @@ __global__ __launch_bounds__(kBlock, FFX_SEARCH_OCCUPANCY) void search_sums(Sear
         for (uint32_t idx = base; idx < end; ++idx) {
             const uint32_t slotIdx = a.nsums - 1 - idx;  // largest sums first
             const uint32_t s = sum_of_slot(a, slotIdx);
-            const uint32_t res = solve_sum_wave(a, tab, s, lane, queues[w], probes[w], segs[w], segOffs[w], primes[threadIdx.x]);
+            const uint32_t res =
+                solve_sum_wave(a, tab, s, lane, queues[w], probes[w], segs[w], segOffs[w], primes[threadIdx.x], dsieve[w]);
             if (lane == 0) {
                 if (!a.bySum) a.slots[slotIdx] = res;
                 else if (s >= a.sumBase && ((s - a.sumBase) >> 1) < a.slotCap) a.slots[(s - a.sumBase) >> 1] = res;
@@ -912,6 +1018,8 @@ struct ffx_search_plan {
     uint32_t tileCap;
     uint32_t *dSlots;     // scratch slots for ffx_search_async
     uint64_t slotCap;
+    uint32_t *dSieveK;    // d-sieve tables (per prime and k; square roots)
+    uint8_t *dSieveSqrt;
     int grid;
     int device;
 };
@@ -991,6 +1099,30 @@ int ffx_search_plan_create(ffx_search_plan **out, uint32_t sumLimitMax)
         }
     if (e == hipSuccess) e = hipMalloc(&pl->dCoprime, kTabWords * 4);
     if (e == hipSuccess) e = hipMemcpy(pl->dCoprime, tab.data(), kTabWords * 4, hipMemcpyHostToDevice);
+    // d-sieve tables: for prime q and k < 32, 2^k mod q, the inverses of
+    // 1 - 4^k (when it is nonzero) and of 2^(k+1); square roots mod q
+    std::vector<uint32_t> sk(kNSievePrimes * 32, 0);
+    std::vector<uint8_t> sq(kSieveSqrtBytes, 0xFF);
+    for (int i = 0; i < kNSievePrimes; ++i) {
+        const uint32_t q = kSievePrimes[i];
+        auto inv = [q](uint32_t x) {
+            for (uint32_t y = 1; y < q; ++y)
+                if (x * y % q == 1) return y;
+            return 0u;
+        };
+        uint32_t pk = 1;
+        for (int k = 0; k < 32; ++k) {
+            const uint32_t A = (1 + q * q - pk * pk % q) % q;
+            sk[i * 32 + k] = pk | (A ? inv(A) : 0u) << 8 | inv(2 * pk % q) << 16 | (A ? 1u : 0u) << 24;
+            pk = 2 * pk % q;
+        }
+        for (uint32_t r = 0; r < q; ++r)
+            if (sq[sieve_sqrt_offset(i) + r * r % q] == 0xFF) sq[sieve_sqrt_offset(i) + r * r % q] = (uint8_t)r;
+    }
+    if (e == hipSuccess) e = hipMalloc(&pl->dSieveK, sk.size() * 4);
+    if (e == hipSuccess) e = hipMemcpy(pl->dSieveK, sk.data(), sk.size() * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&pl->dSieveSqrt, sq.size());
+    if (e == hipSuccess) e = hipMemcpy(pl->dSieveSqrt, sq.data(), sq.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         int rc = hip_fail(e, "search plan upload");
         ffx_search_plan_destroy(pl);
@@ -1010,6 +1142,8 @@ int ffx_search_plan_destroy(ffx_search_plan *plan)
     if (!plan) return FFX_OK;
     (void)hipFree(plan->dSpf);
     (void)hipFree(plan->dCoprime);
+    (void)hipFree(plan->dSieveK);
+    (void)hipFree(plan->dSieveSqrt);
     (void)hipFree(plan->dCounter);
     (void)hipFree(plan->dTile);
     (void)hipFree(plan->dSlots);
@@ -1023,6 +1157,8 @@ static int launch_search(ffx_search_plan *plan, SearchArgs &a, void *stream)
     a.spfLimit = plan->spfLimit;
     a.counter = plan->dCounter;
     a.coprime = plan->dCoprime;
+    a.sieveK = plan->dSieveK;
+    a.sieveSqrt = plan->dSieveSqrt;
     a.grab = plan->grab;
     a.eager = plan->eager;
     hipStream_t st = as_stream(stream);
