@@ -14,6 +14,7 @@
 #include <string>
 #include <vector>
 
+#include "at_exit.h"
 #include "budget.h"
 #include "config.h"
 #include "emission.h"
@@ -157,14 +158,21 @@ int runLeg(const Config &cfg, const std::vector<std::string> &pos)
     // ---- sieve
     auto tSieve = std::chrono::steady_clock::now();
     void *stream = ffx_stream_create();
-    if (!stream) return err("stream", FFX_EIO);
     ffx_sieve_plan *sp = nullptr;
+    void *dMap = nullptr;
+    AtExit release([&] {
+        if (stream) ffx_stream_sync(stream);
+        ffx_free(dMap);
+        ffx_sieve_plan_destroy(sp);
+        ffx_stream_destroy(stream);
+    });
+    if (!stream) return err("stream", FFX_EIO);
     if ((rc = ffx_sieve_plan_create(&sp, mapM))) return err("sieve plan", rc);
     if (cfg.sieveSegmentBytes && (rc = ffx_sieve_plan_set(sp, FFX_SIEVE_SEGMENT_BYTES, cfg.sieveSegmentBytes)))
         return err("sieve tuning", rc);
     if (cfg.sieveBlocksPerCU && (rc = ffx_sieve_plan_set(sp, FFX_SIEVE_BLOCKS_PER_CU, cfg.sieveBlocksPerCU)))
         return err("sieve tuning", rc);
-    void *dMap = ffx_alloc(((mapBytes + 255) & ~255ull));
+    dMap = ffx_alloc(((mapBytes + 255) & ~255ull));
     if (!dMap) return err("map allocation", FFX_ENOMEM);
     std::vector<uint8_t> hostMap;
     if (!cfg.dumpMapFile.empty()) hostMap.resize(mapBytes);
@@ -199,15 +207,27 @@ int runLeg(const Config &cfg, const std::vector<std::string> &pos)
     if (chunkSpan(chunks, &lo, &hi)) {
         if (hi > FFX_MAX_SUM) return err("search range", FFX_ERANGE);
         ffx_search_plan *plan = nullptr;
+        ffx_solution *dOut = nullptr;
+        uint32_t *dCount = nullptr;
+        void *e0 = nullptr, *e1 = nullptr;
+        AtExit releaseSearch([&] {
+            ffx_stream_sync(stream);
+            ffx_event_destroy(e0);
+            ffx_event_destroy(e1);
+            ffx_free(dOut);
+            ffx_free(dCount);
+            ffx_search_plan_destroy(plan);
+        });
         if ((rc = ffx_search_plan_create(&plan, hi))) return err("search plan", rc);
         if ((cfg.searchGrab && (rc = ffx_search_plan_set(plan, FFX_SEARCH_GRAB, cfg.searchGrab))) ||
             (cfg.searchEager && (rc = ffx_search_plan_set(plan, FFX_SEARCH_EAGER_SPLITS, cfg.searchEager))) ||
             (cfg.searchBlocksPerCU && (rc = ffx_search_plan_set(plan, FFX_SEARCH_BLOCKS_PER_CU, cfg.searchBlocksPerCU))))
             return err("search tuning", rc);
         const uint64_t nslots = (uint64_t)(hi - lo) / 2 + 1;
-        ffx_solution *dOut = (ffx_solution *)ffx_alloc(nslots * sizeof(ffx_solution));
-        uint32_t *dCount = (uint32_t *)ffx_alloc(4);
-        void *e0 = ffx_event_create(), *e1 = ffx_event_create();
+        dOut = (ffx_solution *)ffx_alloc(nslots * sizeof(ffx_solution));
+        dCount = (uint32_t *)ffx_alloc(4);
+        e0 = ffx_event_create();
+        e1 = ffx_event_create();
         if (!dOut || !dCount || !e0 || !e1) return err("search allocation", FFX_ENOMEM);
         ffx_event_record(e0, stream);
         if ((rc = ffx_search_async(plan, dMap, mapM, lo, hi, dOut, dCount, stream))) return err("search", rc);
@@ -222,11 +242,6 @@ int runLeg(const Config &cfg, const std::vector<std::string> &pos)
         std::fprintf(stderr, "[ff_sieve] search: sums [%u, %u] (%llu odd), %u solutions, kernels %.3f ms, %.0f sums/s\n",
                      lo, hi, (unsigned long long)nslots, count, searchKernelMs,
                      (double)nslots / (searchKernelMs * 1e-3));
-        ffx_event_destroy(e0);
-        ffx_event_destroy(e1);
-        ffx_free(dOut);
-        ffx_free(dCount);
-        ffx_search_plan_destroy(plan);
     }
     uint64_t ordinal = 0;
     emitChunks(stdout, chunks, sols.data(), sols.size(), &ordinal);
@@ -234,9 +249,6 @@ int runLeg(const Config &cfg, const std::vector<std::string> &pos)
     emitTimes(stdout, (uint64_t)(sieveWall * 1000.0), (uint64_t)(searchWall * 1000.0));
     std::fflush(stdout);
     phase("search phase", searchWall);
-    ffx_free(dMap);
-    ffx_sieve_plan_destroy(sp);
-    ffx_stream_destroy(stream);
     return 0;
 }
 

@@ -24,12 +24,12 @@
 #include <cstring>
 #include <deque>
 #include <fstream>
-#include <functional>
 #include <sstream>
 #include <condition_variable>
 #include <mutex>
 #include <thread>
 
+#include "at_exit.h"
 #include "batch.h"
 #include "budget.h"
 #include "ffx.h"
@@ -42,6 +42,8 @@ namespace {
 using Clock = std::chrono::steady_clock;
 
 double msSince(Clock::time_point t) { return std::chrono::duration<double, std::milli>(Clock::now() - t).count(); }
+
+
 
 int fail(const char *what, int rc)
 {
@@ -158,16 +160,31 @@ int runBatch(const Config &cfg, const BatchOptions &opt)
 
     void *stream = ffx_stream_create();
     void *ev[4] = {ffx_event_create(), ffx_event_create(), ffx_event_create(), ffx_event_create()};
-    if (!stream || !ev[0] || !ev[1] || !ev[2] || !ev[3]) return fail("stream/events", FFX_EIO);
-    int rc;
     ffx_sieve_plan *sp = nullptr;
     ffx_search_plan *qp = nullptr;
+    void *dMap = nullptr;
+    uint32_t *dSlots = nullptr;
+    ffx_interval *dIv = nullptr;
+    ffx_graph *graph = nullptr;
+    AtExit release([&] {
+        if (stream) ffx_stream_sync(stream);
+        ffx_graph_destroy(graph);
+        ffx_free(dIv);
+        ffx_free(dSlots);
+        ffx_free(dMap);
+        ffx_search_plan_destroy(qp);
+        ffx_sieve_plan_destroy(sp);
+        for (void *e : ev) ffx_event_destroy(e);
+        ffx_stream_destroy(stream);
+    });
+    if (!stream || !ev[0] || !ev[1] || !ev[2] || !ev[3]) return fail("stream/events", FFX_EIO);
+    int rc;
     if ((rc = ffx_sieve_plan_create(&sp, M))) return fail("sieve plan", rc);
     if ((rc = ffx_search_plan_create(&qp, U.lastSum() > 5 ? U.lastSum() : 5))) return fail("search plan", rc);
     if ((rc = applyTuning(cfg, sp, qp))) return fail("tuning", rc);
-    void *dMap = ffx_alloc((mapBytes + 255) & ~255ull);
-    uint32_t *dSlots = (uint32_t *)ffx_alloc(U.totalSlots * 4 + 4);
-    ffx_interval *dIv = (ffx_interval *)ffx_alloc(U.iv.size() * sizeof(ffx_interval) + 16);
+    dMap = ffx_alloc((mapBytes + 255) & ~255ull);
+    dSlots = (uint32_t *)ffx_alloc(U.totalSlots * 4 + 4);
+    dIv = (ffx_interval *)ffx_alloc(U.iv.size() * sizeof(ffx_interval) + 16);
     if (!dMap || !dSlots || !dIv) return fail("allocation", FFX_ENOMEM);
     if (!U.iv.empty() && (rc = ffx_copy_h2d(dIv, U.iv.data(), U.iv.size() * sizeof(ffx_interval))))
         return fail("interval upload", rc);
@@ -183,7 +200,6 @@ int runBatch(const Config &cfg, const BatchOptions &opt)
     const double sieveWall = msSince(t0);
 
     // one persistent launch over every interval, captured once, replayed
-    ffx_graph *graph = nullptr;
     if (opt.useGraph && !U.iv.empty()) {
         if ((rc = ffx_graph_begin(stream))) return fail("graph capture", rc);
         rc = ffx_search_intervals_async(qp, dMap, M, dIv, (uint32_t)U.iv.size(), U.totalSlots, U.lastSum(), dSlots,
@@ -257,14 +273,6 @@ int runBatch(const Config &cfg, const BatchOptions &opt)
                 (unsigned long long)lines, (unsigned long long)M, (unsigned long long)mapBytes, sieveMs, searchMs,
                 emitMs, graph ? "true" : "false", (double)requested / ((sieveMs + searchMs) * 1e-3),
                 (double)U.totalSlots / (searchMs * 1e-3), (double)mapBytes / (sieveMs * 1e6));
-    ffx_graph_destroy(graph);
-    ffx_free(dIv);
-    ffx_free(dSlots);
-    ffx_free(dMap);
-    ffx_search_plan_destroy(qp);
-    ffx_sieve_plan_destroy(sp);
-    for (void *e : ev) ffx_event_destroy(e);
-    ffx_stream_destroy(stream);
     return 0;
 }
 
@@ -401,10 +409,29 @@ int runServe(const Config &cfg, const ServeOptions &opt)
 
     int rc;
     ffx_sieve_plan *sp = nullptr;
-    if ((rc = ffx_sieve_plan_create(&sp, M))) return fail("sieve plan", rc);
     std::vector<ffx_search_plan *> qp(S, nullptr);
-    std::vector<void *> st(S), ev(S);
-    std::vector<ffx_interval *> dIv(S), hIv(S);
+    std::vector<void *> st(S, nullptr), ev(S, nullptr);
+    std::vector<ffx_interval *> dIv(S, nullptr), hIv(S, nullptr);
+    void *gst = nullptr, *dMap = nullptr;
+    uint32_t *dSlots = nullptr, *mirror = nullptr;
+    std::vector<void *> growEvents;
+    AtExit release([&] {
+        ffx_device_sync();
+        ffx_free_pinned(mirror);
+        ffx_free(dSlots);
+        ffx_free(dMap);
+        ffx_stream_destroy(gst);
+        for (void *e : growEvents) ffx_event_destroy(e);
+        for (int k = 0; k < S; ++k) {
+            ffx_search_plan_destroy(qp[k]);
+            ffx_stream_destroy(st[k]);
+            ffx_event_destroy(ev[k]);
+            ffx_free(dIv[k]);
+            ffx_free_pinned(hIv[k]);
+        }
+        ffx_sieve_plan_destroy(sp);
+    });
+    if ((rc = ffx_sieve_plan_create(&sp, M))) return fail("sieve plan", rc);
     for (int k = 0; k < S; ++k) {
         if ((rc = ffx_search_plan_create(&qp[k], maxSum))) return fail("search plan", rc);
         if ((rc = applyTuning(cfg, k == 0 ? sp : nullptr, qp[k]))) return fail("tuning", rc);
@@ -414,10 +441,10 @@ int runServe(const Config &cfg, const ServeOptions &opt)
         hIv[k] = (ffx_interval *)ffx_alloc_pinned((uint64_t)ivCap * sizeof(ffx_interval));
         if (!st[k] || !ev[k] || !dIv[k] || !hIv[k]) return fail("stream resources", FFX_ENOMEM);
     }
-    void *gst = ffx_stream_create();
-    void *dMap = ffx_alloc((mapBytesMax + 255) & ~255ull);
-    uint32_t *dSlots = (uint32_t *)ffx_alloc(nslots * 4);
-    uint32_t *mirror = (uint32_t *)ffx_alloc_pinned(nslots * 4);
+    gst = ffx_stream_create();
+    dMap = ffx_alloc((mapBytesMax + 255) & ~255ull);
+    dSlots = (uint32_t *)ffx_alloc(nslots * 4);
+    mirror = (uint32_t *)ffx_alloc_pinned(nslots * 4);
     if (!gst || !dMap || !dSlots || !mirror) return fail("allocation", FFX_ENOMEM);
 
     // The first GiB of the map is sieved before the stream opens (small
@@ -435,7 +462,6 @@ int runServe(const Config &cfg, const ServeOptions &opt)
     const double mapBuildS = std::chrono::duration<double>(Clock::now() - tm0).count();
     uint64_t wantBytes = mapBytesDone, issuedBytes = mapBytesDone;
     std::deque<std::pair<void *, uint64_t>> growQ;  // (event, map bytes done when it fires)
-    std::vector<void *> growEvents;
     auto bytesFor = [&](uint32_t L) { return (std::min(M, uncappedMapValue(L)) >> 4) + 1; };
 
     std::vector<double> doneAt(n, -1.0);
@@ -618,6 +644,10 @@ int runServe(const Config &cfg, const ServeOptions &opt)
         const size_t m = std::min<size_t>(opt.plainSample, n);
         std::vector<double> plat;
         uint32_t *dS = (uint32_t *)ffx_alloc(nslots * 4);
+        AtExit freeS([&] {
+            ffx_device_sync();
+            ffx_free(dS);
+        });
         if (!dS) return fail("plain slots", FFX_ENOMEM);
         if (mapBytesDone < mapBytesMax) {
             if ((rc = ffx_sieve_map_range_async(sp, dMap, mapBytesDone, mapBytesMax, gst)) ||
@@ -658,7 +688,6 @@ int runServe(const Config &cfg, const ServeOptions &opt)
         pp50 = percentile(plat, 0.5);
         pp99 = percentile(plat, 0.99);
         plainRate = (double)plainSums / plainSpan;
-        ffx_free(dS);
     }
     char js[1536];
     std::snprintf(js, sizeof js,
@@ -676,19 +705,6 @@ int runServe(const Config &cfg, const ServeOptions &opt)
                   opt.plainSample, plainSpan, plainRate, pp50, pp99, opt.warm ? "true" : "false", mapBuildS);
     std::fputs(js, stdout);
     if (!opt.jsonOut.empty()) writeFile(opt.jsonOut, js);
-    ffx_free_pinned(mirror);
-    ffx_free(dSlots);
-    ffx_free(dMap);
-    ffx_stream_destroy(gst);
-    for (void *e : growEvents) ffx_event_destroy(e);
-    for (int k = 0; k < S; ++k) {
-        ffx_search_plan_destroy(qp[k]);
-        ffx_stream_destroy(st[k]);
-        ffx_event_destroy(ev[k]);
-        ffx_free(dIv[k]);
-        ffx_free_pinned(hIv[k]);
-    }
-    ffx_sieve_plan_destroy(sp);
     return 0;
 }
 
@@ -726,6 +742,19 @@ int runAutotune(const Config &cfg, const AutotuneOptions &opt)
     uint8_t *dMap = (uint8_t *)ffx_alloc((mapBytes + 255) & ~255ull);
     uint8_t *dChk = (uint8_t *)ffx_alloc((checkBytes + 255) & ~255ull);
     uint32_t *dSlots = (uint32_t *)ffx_alloc(nslots * 4);
+    ffx_sieve_plan *sp = nullptr;
+    ffx_search_plan *qp = nullptr;
+    AtExit release([&] {
+        if (stream) ffx_stream_sync(stream);
+        ffx_search_plan_destroy(qp);
+        ffx_sieve_plan_destroy(sp);
+        ffx_free(dSlots);
+        ffx_free(dChk);
+        ffx_free(dMap);
+        ffx_event_destroy(e0);
+        ffx_event_destroy(e1);
+        ffx_stream_destroy(stream);
+    });
     if (!stream || !e0 || !e1 || !dMap || !dChk || !dSlots) return fail("autotune buffers", FFX_ENOMEM);
     std::vector<uint8_t> chkRef(checkBytes), chk(checkBytes);
     std::vector<uint32_t> slotsRef(nslots), slots(nslots);
@@ -750,7 +779,6 @@ int runAutotune(const Config &cfg, const AutotuneOptions &opt)
         double ms;
     };
     std::vector<SieveTrial> st;
-    ffx_sieve_plan *sp = nullptr;
     if ((rc = ffx_sieve_plan_create(&sp, M))) return fail("sieve plan", rc);
     if ((rc = ffx_sieve_map_async(sp, dChk, checkBytes, stream)) || (rc = ffx_stream_sync(stream)) ||
         (rc = ffx_copy_d2h(chkRef.data(), dChk, checkBytes)))
@@ -787,7 +815,6 @@ int runAutotune(const Config &cfg, const AutotuneOptions &opt)
         double ms;
     };
     std::vector<SearchTrial> qt;
-    ffx_search_plan *qp = nullptr;
     if ((rc = ffx_search_plan_create(&qp, job.hi))) return fail("search plan", rc);
     uint32_t g0 = 0, e0v = 0, b0 = 0;
     ffx_search_plan_get(qp, FFX_SEARCH_GRAB, &g0);
@@ -849,14 +876,6 @@ int runAutotune(const Config &cfg, const AutotuneOptions &opt)
         return 1;
     }
     std::fputs(js.c_str(), stdout);
-    ffx_search_plan_destroy(qp);
-    ffx_sieve_plan_destroy(sp);
-    ffx_free(dSlots);
-    ffx_free(dChk);
-    ffx_free(dMap);
-    ffx_event_destroy(e0);
-    ffx_event_destroy(e1);
-    ffx_stream_destroy(stream);
     return 0;
 }
 
