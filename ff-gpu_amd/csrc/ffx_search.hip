@@ -63,6 +63,9 @@ constexpr int kWavesPerBlock = kBlock / 64;
 constexpr int kQueueCap = 192;     // < 64 left after a drain + 128 from one probe batch
 // Four splits per lane per filter round (measured: 26.4 ms against 28.8 with
 // two at the same occupancy, 36 with one).
+#ifndef FFX_AB1_PRECHECK
+#define FFX_AB1_PRECHECK 1
+#endif
 #ifndef FFX_ROUND_SPLITS
 #define FFX_ROUND_SPLITS 4
 #endif
@@ -528,7 +531,31 @@ __device__ __forceinline__ void drain(const SearchArgs &a, const uint2 *queue, i
         const uint32_t d = ent.x & ((1u << kDBits) - 1), k = ent.x >> kDBits;
         oo = ent.y;
         ee = d << k;
+#if FFX_AB1_PRECHECK
+        // Quick rejection before the full check: with p the smallest prime
+        // factor of d (of o when d = 1), the two pairs AllButOne visits first
+        // for the divisor p of q = d*o -- (p, (q/p) 2^k) and (p 2^k, q/p),
+        // each when its smaller member is at most isqrt(q 2^k) -- have the
+        // terms p + (q/p) 2^k - 2 and p 2^k + q/p - 2.  A composite one (not
+        // the split's own term) makes AllButOne false; both are read at once,
+        // and only splits that survive pay for the factoring of d and o.
+        const uint32_t n1 = d > 1 ? d : oo;
+        uint32_t p = n1 < a.spfLimit ? a.spf[n1 >> 1] : 0u;
+        if (p == 0) p = n1;
+        bool maybe = true;
+        if (p > 1) {
+            const uint64_t q = (uint64_t)d * oo, P = q << k, L = isqrt53(P);
+            const uint64_t c = (uint64_t)((double)q / (double)p);  // exact: p | q < 2^53
+            const uint64_t own = ((uint64_t)d << k) + oo - 2;
+            const uint64_t t1 = (uint64_t)p + (c << k) - 2, t2 = ((uint64_t)p << k) + c - 2;
+            const bool u1 = p <= L && t1 != own && t1 <= a.M, u2 = p <= (L >> k) && t2 != own && t2 <= a.M;
+            const uint32_t b1 = u1 ? map_byte(a.map, t1) : 0u, b2 = u2 ? map_byte(a.map, t2) : 0u;
+            maybe = (!u1 || byte_bit(b1, t1)) && (!u2 || byte_bit(b2, t2));
+        }
+        valid = maybe && ab1(TermTables{a.map, a.M, a.spf, a.spfLimit}, k, d, oo, pr);
+#else
         valid = ab1(TermTables{a.map, a.M, a.spf, a.spfLimit}, k, d, oo, pr);
+#endif
     }
 #ifdef FFX_PROBE_COUNTERS
     if (lane == 0) FFX_COUNT(6, nb);
