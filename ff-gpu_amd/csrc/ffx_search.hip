@@ -63,9 +63,6 @@ constexpr int kWavesPerBlock = kBlock / 64;
 constexpr int kQueueCap = 192;     // < 64 left after a drain + 128 from one probe batch
 // Four splits per lane per filter round (measured: 26.4 ms against 28.8 with
 // two at the same occupancy, 36 with one).
-#ifndef FFX_AB1_PRECHECK
-#define FFX_AB1_PRECHECK 1
-#endif
 #ifndef FFX_ROUND_SPLITS
 #define FFX_ROUND_SPLITS 4
 #endif
@@ -531,31 +528,42 @@ __device__ __forceinline__ void drain(const SearchArgs &a, const uint2 *queue, i
         const uint32_t d = ent.x & ((1u << kDBits) - 1), k = ent.x >> kDBits;
         oo = ent.y;
         ee = d << k;
-#if FFX_AB1_PRECHECK
-        // Quick rejection before the full check: with p the smallest prime
-        // factor of d (of o when d = 1), the two pairs AllButOne visits first
-        // for the divisor p of q = d*o -- (p, (q/p) 2^k) and (p 2^k, q/p),
-        // each when its smaller member is at most isqrt(q 2^k) -- have the
-        // terms p + (q/p) 2^k - 2 and p 2^k + q/p - 2.  A composite one (not
-        // the split's own term) makes AllButOne false; both are read at once,
-        // and only splits that survive pay for the factoring of d and o.
-        const uint32_t n1 = d > 1 ? d : oo;
+        // Quick rejection before the full check: for p the smallest prime
+        // factor of d and p' that of o (of o alone when d = 1), AllButOne
+        // visits the pairs (p, (q/p) 2^k) and (p 2^k, q/p) of q = d*o -- each
+        // when its smaller member is at most isqrt(q 2^k) -- and the same for
+        // p'.  Their terms p + (q/p) 2^k - 2 and p 2^k + q/p - 2 are read at
+        // once; a composite one (not the split's own term) makes AllButOne
+        // false, so only splits that survive pay for factoring d and o.
+        const uint32_t n1 = d > 1 ? d : oo, n2 = d > 1 && oo > 1 ? oo : 1u;
         uint32_t p = n1 < a.spfLimit ? a.spf[n1 >> 1] : 0u;
+        uint32_t p2 = n2 > 1 && n2 < a.spfLimit ? a.spf[n2 >> 1] : 0u;
         if (p == 0) p = n1;
+        if (p2 == 0) p2 = n2;
         bool maybe = true;
         if (p > 1) {
             const uint64_t q = (uint64_t)d * oo, P = q << k, L = isqrt53(P);
-            const uint64_t c = (uint64_t)((double)q / (double)p);  // exact: p | q < 2^53
             const uint64_t own = ((uint64_t)d << k) + oo - 2;
-            const uint64_t t1 = (uint64_t)p + (c << k) - 2, t2 = ((uint64_t)p << k) + c - 2;
-            const bool u1 = p <= L && t1 != own && t1 <= a.M, u2 = p <= (L >> k) && t2 != own && t2 <= a.M;
-            const uint32_t b1 = u1 ? map_byte(a.map, t1) : 0u, b2 = u2 ? map_byte(a.map, t2) : 0u;
-            maybe = (!u1 || byte_bit(b1, t1)) && (!u2 || byte_bit(b2, t2));
+            const bool two = p2 > 1 && p2 != p;
+            const uint64_t c1 = (uint64_t)((double)q / (double)p);  // exact: p | q < 2^53
+            const uint64_t c2 = two ? (uint64_t)((double)q / (double)p2) : 1;
+            const uint64_t t[4] = {(uint64_t)p + (c1 << k) - 2, ((uint64_t)p << k) + c1 - 2,
+                                   (uint64_t)p2 + (c2 << k) - 2, ((uint64_t)p2 << k) + c2 - 2};
+            const bool u[4] = {p <= L, p <= (L >> k), two && p2 <= L, two && p2 <= (L >> k)};
+            uint32_t b[4];
+            bool use[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                use[i] = u[i] && t[i] != own && t[i] <= a.M;
+                b[i] = use[i] ? map_byte(a.map, t[i]) : 0u;
+            }
+#ifdef FFX_PROBE_COUNTERS
+            FFX_COUNT(7, (int)use[0] + (int)use[1] + (int)use[2] + (int)use[3]);
+#endif
+#pragma unroll
+            for (int i = 0; i < 4; ++i) maybe = maybe && (!use[i] || byte_bit(b[i], t[i]));
         }
         valid = maybe && ab1(TermTables{a.map, a.M, a.spf, a.spfLimit}, k, d, oo, pr);
-#else
-        valid = ab1(TermTables{a.map, a.M, a.spf, a.spfLimit}, k, d, oo, pr);
-#endif
     }
 #ifdef FFX_PROBE_COUNTERS
     if (lane == 0) FFX_COUNT(6, nb);
