@@ -532,9 +532,10 @@ __device__ __forceinline__ void drain(const SearchArgs &a, const uint2 *queue, i
         // factor of d and p' that of o (of o alone when d = 1), AllButOne
         // visits the pairs (p, (q/p) 2^k) and (p 2^k, q/p) of q = d*o -- each
         // when its smaller member is at most isqrt(q 2^k) -- and the same for
-        // p'.  Their terms p + (q/p) 2^k - 2 and p 2^k + q/p - 2 are read at
-        // once; a composite one (not the split's own term) makes AllButOne
-        // false, so only splits that survive pay for factoring d and o.
+        // p'.  Their terms p + (q/p) 2^k - 2 and p 2^k + q/p - 2 are read in
+        // two rounds (p's pair, then p''s pair for the splits still in); a
+        // composite one (not the split's own term) makes AllButOne false, so
+        // only splits that survive pay for factoring d and o.
         const uint32_t n1 = d > 1 ? d : oo, n2 = d > 1 && oo > 1 ? oo : 1u;
         uint32_t p = n1 < a.spfLimit ? a.spf[n1 >> 1] : 0u;
         uint32_t p2 = n2 > 1 && n2 < a.spfLimit ? a.spf[n2 >> 1] : 0u;
@@ -550,18 +551,25 @@ __device__ __forceinline__ void drain(const SearchArgs &a, const uint2 *queue, i
             const uint64_t t[4] = {(uint64_t)p + (c1 << k) - 2, ((uint64_t)p << k) + c1 - 2,
                                    (uint64_t)p2 + (c2 << k) - 2, ((uint64_t)p2 << k) + c2 - 2};
             const bool u[4] = {p <= L, p <= (L >> k), two && p2 <= L, two && p2 <= (L >> k)};
+            // p's pair first; p''s only when p's passes (half the reads)
             uint32_t b[4];
             bool use[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                use[i] = u[i] && t[i] != own && t[i] <= a.M;
+            for (int i = 0; i < 4; ++i) use[i] = u[i] && t[i] != own && t[i] <= a.M;
+#pragma unroll
+            for (int i = 0; i < 2; ++i) b[i] = use[i] ? map_byte(a.map, t[i]) : 0u;
+#pragma unroll
+            for (int i = 0; i < 2; ++i) maybe = maybe && (!use[i] || byte_bit(b[i], t[i]));
+#pragma unroll
+            for (int i = 2; i < 4; ++i) {
+                use[i] = use[i] && maybe;
                 b[i] = use[i] ? map_byte(a.map, t[i]) : 0u;
             }
+#pragma unroll
+            for (int i = 2; i < 4; ++i) maybe = maybe && (!use[i] || byte_bit(b[i], t[i]));
 #ifdef FFX_PROBE_COUNTERS
             FFX_COUNT(7, (int)use[0] + (int)use[1] + (int)use[2] + (int)use[3]);
 #endif
-#pragma unroll
-            for (int i = 0; i < 4; ++i) maybe = maybe && (!use[i] || byte_bit(b[i], t[i]));
         }
         valid = maybe && ab1(TermTables{a.map, a.M, a.spf, a.spfLimit}, k, d, oo, pr);
     }
