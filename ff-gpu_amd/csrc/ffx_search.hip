@@ -533,9 +533,9 @@ __device__ __forceinline__ void drain(const SearchArgs &a, const uint2 *queue, i
         // visits the pairs (p, (q/p) 2^k) and (p 2^k, q/p) of q = d*o -- each
         // when its smaller member is at most isqrt(q 2^k) -- and the same for
         // p'.  Their terms p + (q/p) 2^k - 2 and p 2^k + q/p - 2 are read in
-        // two rounds (p's pair, then p''s pair for the splits still in); a
-        // composite one (not the split's own term) makes AllButOne false, so
-        // only splits that survive pay for factoring d and o.
+        // three rounds (below); a composite one (not the split's own term)
+        // makes AllButOne false, so only splits that survive pay for
+        // factoring d and o.
         const uint32_t n1 = d > 1 ? d : oo, n2 = d > 1 && oo > 1 ? oo : 1u;
         uint32_t p = n1 < a.spfLimit ? a.spf[n1 >> 1] : 0u;
         uint32_t p2 = n2 > 1 && n2 < a.spfLimit ? a.spf[n2 >> 1] : 0u;
@@ -551,15 +551,18 @@ __device__ __forceinline__ void drain(const SearchArgs &a, const uint2 *queue, i
             const uint64_t t[4] = {(uint64_t)p + (c1 << k) - 2, ((uint64_t)p << k) + c1 - 2,
                                    (uint64_t)p2 + (c2 << k) - 2, ((uint64_t)p2 << k) + c2 - 2};
             const bool u[4] = {p <= L, p <= (L >> k), two && p2 <= L, two && p2 <= (L >> k)};
-            // p's pair first; p''s only when p's passes (half the reads)
+            // one term of p's pair, then its other term, then p''s pair, each
+            // read only while the split is still in (fewer reads: the kernel
+            // is bound by them)
             uint32_t b[4];
             bool use[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) use[i] = u[i] && t[i] != own && t[i] <= a.M;
-#pragma unroll
-            for (int i = 0; i < 2; ++i) b[i] = use[i] ? map_byte(a.map, t[i]) : 0u;
-#pragma unroll
-            for (int i = 0; i < 2; ++i) maybe = maybe && (!use[i] || byte_bit(b[i], t[i]));
+            b[0] = use[0] ? map_byte(a.map, t[0]) : 0u;
+            maybe = !use[0] || byte_bit(b[0], t[0]);
+            use[1] = use[1] && maybe;
+            b[1] = use[1] ? map_byte(a.map, t[1]) : 0u;
+            maybe = maybe && (!use[1] || byte_bit(b[1], t[1]));
 #pragma unroll
             for (int i = 2; i < 4; ++i) {
                 use[i] = use[i] && maybe;
