@@ -284,7 +284,9 @@ def main():
     sieve_gbs = map_bytes / (sieve_avg * 1e-3) / 1e9
     roof_sieve = {"bound": "hbm", "achieved": round(sieve_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                   "frac": round(sieve_gbs / HBM_PEAK_GBS, 4), "traffic": traffic_for("sieve_segments", args.workload),
-                  "kernel": "sieve_segments", "algorithmic": f"{map_bytes} map bytes written once"}
+                  "kernel": "sieve_segments", "algorithmic": f"{map_bytes} map bytes written once",
+                  "note": "HBM is not what binds the sieve: its strikes are LDS atomics and its issue runs near "
+                          "the VALU peak (VALU wave-instructions per CU per clock in profiles/pmc_plain_2p20.md)"}
 
     for b in (dmap, slots, out, cnt):
         b.close()
