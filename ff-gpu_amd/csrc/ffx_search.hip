@@ -27,15 +27,17 @@
 // lists only the classes of d mod 30 that can hold a valid split (the
 // Power2Odd family, both probe values coprime to 15).  Every listed split
 // first goes through the filters that need no map read: the Power2Even
-// family test and a residue filter (a probe value with a prime factor in
-// 7..31 is composite; the residues of both probe values come from those of
-// d, o and 2^k).  The few survivors
-// are compacted into a per-wave LDS probe queue and probed 128 at a time, so
-// the random map reads run on full waves; splits that pass both PTP probes
-// go to a second queue for AllButOne, resolved 64 at a time, one per lane.
-// While a sum is young both queues are emptied every round (early exit at
-// two valid splits).  Terms d, o < s are factored with a smallest-prime-
-// factor table, so no 64-bit trial division runs on the device.
+// family test, a residue filter (a probe value with a prime factor in 7..31
+// is composite; the residues of both probe values come from those of d, o
+// and 2^k) and, in long k-segments, a sieve over d by the primes 37..131.
+// The few survivors are compacted into a per-wave LDS probe queue and probed
+// 128 at a time, so the random map reads run on full waves; splits that pass
+// both PTP probes go to a second queue for AllButOne, resolved 64 at a time,
+// one per lane, after a pre-check that reads the terms of the first divisor
+// pairs and rejects almost all of them.  While a sum is young both queues
+// are emptied every round (early exit at two valid splits).  Terms d, o < s
+// are factored with a smallest-prime-factor table, so no 64-bit trial
+// division runs on the device.
 #include <cmath>
 #include <cstring>
 #include <mutex>
