@@ -36,11 +36,10 @@ namespace {
 // Defaults measured on MI355X (ff_sieve --autotune and direct timing):
 // maps whose sieving primes stay below 2^19 (M < 2^38) are fastest with
 // 32 KiB segments and four 1024-thread workgroups per CU (two resident with
-// the 40 KiB of patterns): 4 GiB map 12.9 ms against 13.9 with 64 KiB.
+// the 40 KiB of patterns): 4 GiB map 9.45 ms against 10.15 with 64 KiB.
 // Larger maps, where most primes strike a segment at most a few times and
 // each segment pays every prime's first-multiple computation, are faster
-// with 64 KiB segments, two per CU: 70 GB map 316 against 346 ms, 256 GiB
-// map 1.65 against 1.99 s.
+// with 64 KiB segments, two per CU: 16 GiB map 43.4 against 46.1 ms.
 constexpr uint32_t kSmallMapSegBytes = 32768, kSmallMapBlocksPerCU = 4;
 constexpr uint32_t kLargeMapSegBytes = 65536, kLargeMapBlocksPerCU = 2;
 constexpr uint64_t kLargeMapFrom = 1ull << 38;
@@ -131,8 +130,8 @@ __device__ __forceinline__ uint32_t to_map_order(uint32_t v)
 // which halves the registers per carried prime.  32 KiB segments run eight
 // waves per SIMD (64 registers per lane), 64 KiB segments four (128).
 // Measured on MI355X (4 GiB / 16 GiB / 70 GB maps): 32 KiB with 12 re-read
-// carries 11.4 ms (8 held: 11.8); 64 KiB with 32 held 53.9 ms on 16 GiB
-// (64 re-read: 55.7) but 278 ms on 70 GB, where 64 re-read take 263.
+// carries 9.45 ms (8: 9.7, none: 10.1); 64 KiB with 32 held 43.4 ms on
+// 16 GiB (64 re-read: 44.9) but 214 ms on 70 GB, where 64 re-read take 202.
 #ifndef FFX_FILL_UNROLL
 #define FFX_FILL_UNROLL 4
 #endif
