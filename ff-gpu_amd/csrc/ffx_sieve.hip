@@ -156,7 +156,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kSegBy
     for (uint32_t i = tid; i < kPatWords; i += kThreads) pat[i] = a.patterns[i];
 
     // Per-thread pattern cursors: byte offset 4 * ((segWord0 + tid) mod
-    // period), advanced by kThreads words per fill step and by a segment per
+    // period), advanced by kThreads words per fill step and by kSegWords per
     // segment, so no division runs inside the loops.  Kept in bytes and
     // wrapped with an unsigned minimum, a cursor costs three instructions
     // per word (4 GiB map 9.43 -> 9.07 ms against index cursors).
@@ -257,6 +257,16 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kSegBy
                         const uint32_t stepBytes = 4u * p, endBytes = ((limit + 31) >> 5) << 2;
                         char *segBytes = reinterpret_cast<char *>(seg);
                         uint32_t ad = (j >> 5) << 2;
+                        if (kSegBytes == 32768) {
+                            // two strikes per step while both are below the
+                            // end (4 GiB map 9.08 -> 8.93 ms; the 64 KiB
+                            // kernels measured slower with it)
+                            const uint32_t endPair = endBytes > stepBytes ? endBytes - stepBytes : 0u;
+                            for (; ad < endPair; ad += 2 * stepBytes) {
+                                atomicOr(reinterpret_cast<uint32_t *>(segBytes + ad), mask);
+                                atomicOr(reinterpret_cast<uint32_t *>(segBytes + ad + stepBytes), mask);
+                            }
+                        }
                         for (; ad < endBytes; ad += stepBytes)
                             atomicOr(reinterpret_cast<uint32_t *>(segBytes + ad), mask);
                         je = (ad << 3) | (j & 31);
