@@ -279,16 +279,16 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kSegBy
                     if (limit == kSegOdds && (d < p || (hl == 0 && first >= kSegOdds))) coopNext[pi] = d;
                 }
             }
-            // after the last strike j >= limit; with limit == kSegOdds it is
-            // the next segment's first strike (a segment cut short by
-            // oddLimit is the last one the run strikes)
-            // re-read primes: the empty asm keeps the compiler from hoisting
-            // the loads out of the segment loop (holding them after all)
-            // The pointer is then cast back to the global address space: a
-            // flat load would also count on the LDS counter, and its wait
-            // inside the strike loop would wait out every ds_or as well
-            // (4 GiB map 10.2 -> 9.9 ms, 70 GB map 241 -> 215 ms with the
-            // one-ahead prefetch below).
+            // Carried lane primes.  After a prime's last strike j >= limit;
+            // with limit == kSegOdds, j - kSegOdds is the next segment's first
+            // strike (a segment cut short by oddLimit is the last one the run
+            // strikes).  Re-read primes: the empty asm keeps the compiler
+            // from hoisting their loads out of the segment loop (holding them
+            // after all); the pointer is then cast back to the global address
+            // space, since a flat load also counts on the LDS counter and its
+            // wait inside the strike loop would wait out every ds_or as well
+            // (4 GiB map 10.2 -> 9.9 ms, 70 GB map 241 -> 215 ms together with
+            // the one-ahead prefetch).
             const uint32_t *cprGeneric = a.primes + a.ncoop + tid;
             if (!kHoldPrimes) asm volatile("" : "+v"(cprGeneric));
             const __attribute__((address_space(1))) uint32_t *cpr =
