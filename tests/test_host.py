@@ -13,6 +13,18 @@ from conftest import GOLDEN, ROOT, golden_headers, golden_solutions, run_ffref, 
 HC = os.path.join(ROOT, "ff-gpu_amd", "bin", "ff_hostcheck")
 
 
+@pytest.fixture(autouse=True, scope="module", params=["plain", "sanitized"])
+def hostcheck_binary(request):
+    """Every check runs twice: with the plain build and with the build under
+    AddressSanitizer and UBSan, which aborts on any finding."""
+    global HC
+    name = "ff_hostcheck" if request.param == "plain" else "ff_hostcheck_san"
+    HC = os.path.join(ROOT, "ff-gpu_amd", "bin", name)
+    if not os.path.exists(HC):
+        pytest.skip(f"{name} not built (make -C ff-gpu_amd)")
+    yield HC
+
+
 def hc(*args, env=None, check=True):
     p = subprocess.run([HC] + [str(a) for a in args], capture_output=True, text=True, env=env)
     if check:
