@@ -27,7 +27,7 @@
 // lists only the classes of d mod 30 that can hold a valid split (the
 // Power2Odd family, both probe values coprime to 15).  Every listed split
 // first goes through the filters that need no map read: the Power2Even
-// family test, a residue filter (a probe value with a prime factor in 7..31
+// family test, a residue filter (a probe value with a prime factor in 7..23
 // is composite; the residues of both probe values come from those of d, o
 // and 2^k) and, in long k-segments, a sieve over d by the primes 37..131.
 // The few survivors are compacted into a per-wave LDS probe queue and probed
@@ -91,12 +91,14 @@ constexpr uint32_t kBlocksPerCU = 4;  // default persistent grid: CUs x this (al
 // the 32-bit residues of d, o and 2^k with 32-bit products.
 constexpr uint32_t kMods[5] = {3 * 5 * 7 * 11 * 13, 17 * 19 * 23, 29 * 31, 37 * 41, 43 * 47};
 constexpr uint32_t kModTopPrime[5] = {13, 23, 31, 41, 47};
-// Three moduli measured fastest on MI355X with the mod-30 split classes
-// ([5, 2^20) search: 23.6 ms with one, 20.2 with two, 19.7 with three to
-// five; [5, 2^21): 69.7 ms with three).  The classes already exclude 3 and
-// 5; the tables add 7..31.
+// Two moduli measured fastest on MI355X with the mod-30 split classes, the
+// sieve over d and the AllButOne pre-check: [5, 2^20) search 11.4 ms with
+// two, 11.9 with one or three; [5, 2^21) 45.2 ms with two, 46.3 with three,
+// 48.9 with one (before the sieve over d, three were fastest: 19.7 ms
+// against 20.2 with two).  The classes already exclude 3 and 5; the tables
+// add 7..23.
 #ifndef FFX_RESIDUE_MODULI
-#define FFX_RESIDUE_MODULI 3
+#define FFX_RESIDUE_MODULI 2
 #endif
 constexpr int kNMods = FFX_RESIDUE_MODULI;
 static_assert(kNMods >= 1 && kNMods <= 5, "1..5 residue moduli");
@@ -142,7 +144,7 @@ struct SearchArgs {
     uint32_t nsums;           // slots to fill: odd sums first, first+2, ... or all intervals
     uint32_t *slots;
     uint32_t *counter;
-    const uint32_t *coprime;  // kTabWords words: the three residue tables
+    const uint32_t *coprime;  // kTabWords words: the residue tables
     const ffx_interval *iv;   // when niv > 0: slot g is sum iv[i].first + 2(g - iv[i].slotOffset)
     uint32_t niv;
     uint32_t grab;            // sums taken per atomic
