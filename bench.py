@@ -279,9 +279,9 @@ def main():
                 "traffic_gbs": round(traffic / (search_avg * 1e-3) / 1e9, 1) if traffic else None,
                 "note": "frac above 1: the kernel's residue filter and family-aware split lists skip most of the "
                         "reference's probes, so it reads far fewer bytes (traffic, measured by rocprofv3 "
-                        "FETCH_SIZE/WRITE_SIZE) than the reference-order work implies; its map reads run below "
-                        "the device's random-read ceiling (random_read_roof) and VALU issue near its peak binds "
-                        "(VALU wave-instructions per CU per clock in profiles/pmc_plain_2p20.md)"}
+                        "FETCH_SIZE/WRITE_SIZE) than the reference-order work implies; its map reads run near "
+                        "the device's random-read ceiling (random_read_roof) with VALU issue near its peak "
+                        "(VALU wave-instructions per CU per clock in profiles/pmc_plain_2p20.md): both bind"}
     else:
         roof = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
                 "traffic": None, "kernel": "search_sums", "algorithmic": "reference probe count missing"}
