@@ -182,26 +182,6 @@ __device__ __forceinline__ uint32_t sum_of_slot(const SearchArgs &a, uint32_t g)
 
 // ---------------------------------------------------------------- primality
 
-// Montgomery product a*b/2^64 mod n for odd n < 2^63, a, b < n.
-__device__ __forceinline__ uint64_t mont_mul(uint64_t a, uint64_t b, uint64_t n, uint64_t nneg)
-{
-    const uint64_t lo = a * b, hi = __umul64hi(a, b);
-    const uint64_t m = lo * nneg;
-    const uint64_t t = hi + __umul64hi(m, n) + (lo != 0);
-    return t >= n ? t - n : t;
-}
-
-// a*b mod n for a, b < n < 2^52: the quotient from doubles is off by at
-// most one, the remainder is exact in wrapping 64-bit arithmetic.
-__device__ __forceinline__ uint64_t mulmod52(uint64_t a, uint64_t b, uint64_t n)
-{
-    const uint64_t q = (uint64_t)((double)a * (double)b / (double)n);
-    int64_t r = (int64_t)(a * b - q * n);
-    while (r < 0) r += (int64_t)n;
-    while (r >= (int64_t)n) r -= (int64_t)n;
-    return (uint64_t)r;
-}
-
 // a*b mod n for a, b < n < 2^50, all exact integers held in doubles: h is
 // the rounded product and l = a*b - h exactly (one fused multiply-add); the
 // quotient floor(h/n) from the reciprocal is off by at most one (relative
@@ -219,10 +199,16 @@ __device__ __forceinline__ double mulmod_f64(double a, double b, double n, doubl
     return r;
 }
 
-// Miller-Rabin for odd 17 <= n < 2^50 in double-precision arithmetic
-// (same bases and base counts as mr_is_prime).
-__device__ bool mr_is_prime_f64(uint64_t n)
+// Deterministic Miller-Rabin with ff_seg's bases {2,3,5,7,11,13,17} and
+// base count by magnitude (AskMillerRabin, segmentedSieve.C:364-387), for
+// odd 17 <= n < 2^50, in double-precision arithmetic: a modular product is
+// about ten fused floating-point operations.  Every value the search tests
+// is below 2^50 for sums up to FFX_MAX_SUM (2^26 - 1): the probe values
+// 2^k + d*o - 2 <= s^2/16 + 4 and d + o*2^k - 2 < s, the AllButOne terms
+// f + P/f <= P/2 + 2 with P <= s^2/4, and s - 2, s/3 and the terms of s.
+__device__ __noinline__ bool mr_is_prime(uint64_t n)
 {
+    FFX_COUNT(8, 1);
     const double nd = (double)n, ninv = 1.0 / nd, mone = nd - 1.0;
     const uint32_t r = (uint32_t)__builtin_ctzll(n - 1);
     const uint64_t d = (n - 1) >> r;
@@ -243,60 +229,6 @@ __device__ bool mr_is_prime_f64(uint64_t n)
                 break;
             }
             if (y == 1.0) return false;
-        }
-        if (!pass) return false;
-    }
-    return true;
-}
-
-// Deterministic Miller-Rabin with ff_seg's bases {2,3,5,7,11,13,17} and
-// base count by magnitude (AskMillerRabin, segmentedSieve.C:364-387).
-// n odd, 17 <= n < 2^63.  Below 2^50 (every probe value of a sum below
-// 2^26) the double-precision path runs: a modular product is about ten
-// fused floating-point operations against ~30 integer ones in Montgomery
-// form.
-__device__ __noinline__ bool mr_is_prime(uint64_t n)
-{
-    FFX_COUNT(8, 1);
-    if (n < (1ull << 50)) return mr_is_prime_f64(n);
-    uint64_t inv = n;  // n * n == 1 mod 8
-#pragma unroll
-    for (int i = 0; i < 5; ++i) inv *= 2 - n * inv;
-    const uint64_t nneg = 0 - inv;
-    uint64_t one, r2;  // 2^64 mod n, 2^128 mod n
-    if (n < (1ull << 52)) {
-        const uint64_t t = n > (1ull << 32) ? (1ull << 32) : (1ull << 32) % n;
-        one = mulmod52(t, t, n);
-        r2 = mulmod52(one, one, n);
-    } else {
-        one = (0 - n) % n;
-        r2 = one;
-        for (int i = 0; i < 64; ++i) {
-            r2 <<= 1;
-            if (r2 >= n) r2 -= n;
-        }
-    }
-    const uint64_t mone = n - one;
-    const uint32_t r = (uint32_t)__builtin_ctzll(n - 1);
-    const uint64_t d = (n - 1) >> r;
-    const int nb = n < 3215031751ull ? 4 : n < 2152302898747ull ? 5 : n < 3474749660383ull ? 6 : 7;
-    const uint32_t bases[7] = {2, 3, 5, 7, 11, 13, 17};
-    for (int bi = 0; bi < nb; ++bi) {
-        uint64_t b = mont_mul(bases[bi], r2, n, nneg);
-        uint64_t y = one;
-        for (uint64_t e = d; e; e >>= 1) {
-            if (e & 1) y = mont_mul(y, b, n, nneg);
-            b = mont_mul(b, b, n, nneg);
-        }
-        if (y == one || y == mone) continue;
-        bool pass = false;
-        for (uint32_t j = 1; j < r; ++j) {
-            y = mont_mul(y, y, n, nneg);
-            if (y == mone) {
-                pass = true;
-                break;
-            }
-            if (y == one) return false;
         }
         if (!pass) return false;
     }
