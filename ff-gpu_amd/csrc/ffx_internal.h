@@ -21,6 +21,11 @@ int hip_fail(hipError_t e, const char *what);
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Once per device, with the device current (ffx_init): takes the device
+// address of the allocation-free search's scratch array and fills its
+// tables (ffx_search.hip).
+int noalloc_setup(int device);
+
 }  // namespace ffx
 
 #define FFX_HIP(call)                                                  \

@@ -47,7 +47,7 @@ int ffx_init(int device)
     if (device < 0 || device >= n) return fail(FFX_ENODEV, "ffx_init: device %d of %d", device, n);
     FFX_HIP(hipSetDevice(device));
     FFX_HIP(hipFree(nullptr));  // create the context now, not inside a timed region
-    return FFX_OK;
+    return noalloc_setup(device);
 }
 
 int ffx_device_count(int *count)

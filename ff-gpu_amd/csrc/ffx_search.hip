@@ -39,6 +39,7 @@
 // are emptied every round (early exit at two valid splits).  Terms d, o < s
 // are factored with a smallest-prime-factor table, so no 64-bit trial
 // division runs on the device.
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -1038,6 +1039,63 @@ struct ffx_search_plan {
     int device;
 };
 
+// The plan's read-only tables, built on the host: the smallest prime factor
+// of each odd n < spfLimit (index n>>1; 0 when n is 1 or prime), the
+// residue filter's coprimality bits (bit r of modulus m set when
+// gcd(r, m) == 1), and the sieve over d's tables: for sieve prime q and
+// k < 32, 2^k mod q | (1-4^k)^-1 << 8 | (2^(k+1))^-1 << 16 | (1-4^k != 0)
+// << 24, and a square root mod q of every quadratic residue (0xFF for the
+// others).
+struct SearchTables {
+    uint32_t spfLimit;
+    std::vector<uint16_t> spf;
+    std::vector<uint32_t> coprime;
+    std::vector<uint32_t> sieveK;
+    std::vector<uint8_t> sieveSqrt;
+};
+
+static SearchTables build_search_tables(uint32_t sumLimitMax)
+{
+    SearchTables t;
+    t.spfLimit = sumLimitMax + 2;
+    t.spf.assign((t.spfLimit >> 1) + 1, 0);
+    for (uint64_t p = 3; p * p < t.spfLimit; p += 2) {
+        if (t.spf[p >> 1]) continue;
+        for (uint64_t m = p * p; m < t.spfLimit; m += 2 * p)
+            if (!t.spf[m >> 1]) t.spf[m >> 1] = (uint16_t)p;
+    }
+    t.coprime.assign(kTabWords, 0);
+    for (int i = 0; i < kNMods; ++i)
+        for (uint32_t r = 0; r < kMods[i]; ++r) {
+            uint32_t g = r, m = kMods[i];
+            while (m) {
+                const uint32_t tmp = g % m;
+                g = m;
+                m = tmp;
+            }
+            if (g == 1) t.coprime[tab_offset(i) + r / 32] |= 1u << (r % 32);
+        }
+    t.sieveK.assign(kNSievePrimes * 32, 0);
+    t.sieveSqrt.assign(kSieveSqrtBytes, 0xFF);
+    for (int i = 0; i < kNSievePrimes; ++i) {
+        const uint32_t q = kSievePrimes[i];
+        auto inv = [q](uint32_t x) {
+            for (uint32_t y = 1; y < q; ++y)
+                if (x * y % q == 1) return y;
+            return 0u;
+        };
+        uint32_t pk = 1;
+        for (int k = 0; k < 32; ++k) {
+            const uint32_t A = (1 + q * q - pk * pk % q) % q;
+            t.sieveK[i * 32 + k] = pk | (A ? inv(A) : 0u) << 8 | inv(2 * pk % q) << 16 | (A ? 1u : 0u) << 24;
+            pk = 2 * pk % q;
+        }
+        for (uint32_t r = 0; r < q; ++r)
+            if (t.sieveSqrt[sieve_sqrt_offset(i) + r * r % q] == 0xFF) t.sieveSqrt[sieve_sqrt_offset(i) + r * r % q] = (uint8_t)r;
+    }
+    return t;
+}
+
 using namespace ffx;
 
 static int ensure_scratch(ffx_search_plan *pl, uint64_t nslots, bool slots)
@@ -1060,6 +1118,63 @@ static int ensure_scratch(ffx_search_plan *pl, uint64_t nslots, bool slots)
     return FFX_OK;
 }
 
+// ------------------------------------------------------ allocation-free search
+//
+// ffx_search_records_noalloc runs the search of ffx_search_records with no
+// allocation at call time: its work counter, tables and slots live in one
+// file-scope device array, `scratch`, of kNaBytes (6.3 MB: sums up to
+// FFX_NOALLOC_MAX_SUM).  Each device holds its own copy at its own address;
+// the host takes that address once per device at ffx_init (with the device
+// current) and passes it to the kernels.  The array's name is never used as
+// a device address in host code.
+constexpr uint64_t na_align(uint64_t x) { return (x + 255) & ~255ull; }
+constexpr uint32_t kNaSpfLimit = FFX_NOALLOC_MAX_SUM + 2;
+constexpr uint64_t kNaSlots = (FFX_NOALLOC_MAX_SUM - 5) / 2 + 1;
+constexpr uint64_t kNaCounterOff = 0;  // 16 u32 work counters
+constexpr uint64_t kNaCoprimeOff = 256;
+constexpr uint64_t kNaSieveKOff = kNaCoprimeOff + na_align(kTabWords * 4ull);
+constexpr uint64_t kNaSqrtOff = kNaSieveKOff + na_align(kNSievePrimes * 32ull * 4);
+constexpr uint64_t kNaSpfOff = kNaSqrtOff + na_align(kSieveSqrtBytes);
+constexpr uint64_t kNaSlotsOff = kNaSpfOff + na_align(((kNaSpfLimit >> 1) + 1ull) * 2);
+constexpr uint64_t kNaBytes = kNaSlotsOff + na_align(kNaSlots * 4);
+static_assert(kNaBytes < (8ull << 20), "the allocation-free scratch stays below 8 MiB");
+
+static __device__ __align__(256) uint8_t scratch[kNaBytes];
+
+constexpr int kNaMaxDevices = 64;
+static std::atomic<uint8_t *> g_naScratch[kNaMaxDevices];  // scratch's device address, per device
+static uint32_t g_naCus[kNaMaxDevices];
+
+namespace ffx {
+
+// Once per device, with that device current: the address of `scratch` on
+// it, and the search tables for sums up to FFX_NOALLOC_MAX_SUM copied in.
+int noalloc_setup(int device)
+{
+    static std::mutex mu;
+    std::lock_guard<std::mutex> lock(mu);
+    if (device < 0 || device >= kNaMaxDevices) return fail(FFX_ENODEV, "no-alloc set-up: device %d", device);
+    if (g_naScratch[device].load(std::memory_order_acquire)) return FFX_OK;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    uint8_t *base = static_cast<uint8_t *>(p);
+    const SearchTables t = build_search_tables(FFX_NOALLOC_MAX_SUM);
+    if (t.spfLimit != kNaSpfLimit || t.spf.size() * 2 > kNaSlotsOff - kNaSpfOff)
+        return fail(FFX_EIO, "no-alloc set-up: table layout");
+    FFX_HIP(hipMemcpy(base + kNaCoprimeOff, t.coprime.data(), t.coprime.size() * 4, hipMemcpyHostToDevice));
+    FFX_HIP(hipMemcpy(base + kNaSieveKOff, t.sieveK.data(), t.sieveK.size() * 4, hipMemcpyHostToDevice));
+    FFX_HIP(hipMemcpy(base + kNaSqrtOff, t.sieveSqrt.data(), t.sieveSqrt.size(), hipMemcpyHostToDevice));
+    FFX_HIP(hipMemcpy(base + kNaSpfOff, t.spf.data(), t.spf.size() * 2, hipMemcpyHostToDevice));
+    int cus = 0;
+    FFX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    g_naCus[device] = cus > 0 ? (uint32_t)cus : 256;
+    g_naScratch[device].store(base, std::memory_order_release);
+    return FFX_OK;
+}
+
+}  // namespace ffx
+
 extern "C" {
 
 #ifdef FFX_PROBE_COUNTERS
@@ -1081,62 +1196,23 @@ int ffx_search_plan_create(ffx_search_plan **out, uint32_t sumLimitMax)
     *out = nullptr;
     if (sumLimitMax > FFX_MAX_SUM) return fail(FFX_ERANGE, "search plan: sumLimitMax %u > %u", sumLimitMax, FFX_MAX_SUM);
     if (sumLimitMax < 5) sumLimitMax = 5;
-    // smallest prime factor of odd n < spfLimit (0 when n is 1 or prime)
-    const uint32_t spfLimit = sumLimitMax + 2;
-    std::vector<uint16_t> spf((spfLimit >> 1) + 1, 0);
-    for (uint64_t p = 3; p * p < spfLimit; p += 2) {
-        if (spf[p >> 1]) continue;
-        for (uint64_t m = p * p; m < spfLimit; m += 2 * p)
-            if (!spf[m >> 1]) spf[m >> 1] = (uint16_t)p;
-    }
+    const SearchTables t = build_search_tables(sumLimitMax);
     ffx_search_plan *pl = new (std::nothrow) ffx_search_plan();
     if (!pl) return fail(FFX_ENOMEM, "search plan: host alloc");
     pl->sumLimitMax = sumLimitMax;
-    pl->spfLimit = spfLimit;
+    pl->spfLimit = t.spfLimit;
     hipError_t e = hipGetDevice(&pl->device);
     int cus = 0;
     if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, pl->device);
-    if (e == hipSuccess) e = hipMalloc(&pl->dSpf, spf.size() * 2);
-    if (e == hipSuccess) e = hipMemcpy(pl->dSpf, spf.data(), spf.size() * 2, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&pl->dSpf, t.spf.size() * 2);
+    if (e == hipSuccess) e = hipMemcpy(pl->dSpf, t.spf.data(), t.spf.size() * 2, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMalloc(&pl->dCounter, 64);
-    // coprimality bit tables of the residue filter
-    std::vector<uint32_t> tab(kTabWords, 0);
-    for (int t = 0; t < kNMods; ++t)
-        for (uint32_t r = 0; r < kMods[t]; ++r) {
-            uint32_t g = r, m = kMods[t];
-            while (m) {
-                const uint32_t tmp = g % m;
-                g = m;
-                m = tmp;
-            }
-            if (g == 1) tab[tab_offset(t) + r / 32] |= 1u << (r % 32);
-        }
     if (e == hipSuccess) e = hipMalloc(&pl->dCoprime, kTabWords * 4);
-    if (e == hipSuccess) e = hipMemcpy(pl->dCoprime, tab.data(), kTabWords * 4, hipMemcpyHostToDevice);
-    // d-sieve tables: for prime q and k < 32, 2^k mod q, the inverses of
-    // 1 - 4^k (when it is nonzero) and of 2^(k+1); square roots mod q
-    std::vector<uint32_t> sk(kNSievePrimes * 32, 0);
-    std::vector<uint8_t> sq(kSieveSqrtBytes, 0xFF);
-    for (int i = 0; i < kNSievePrimes; ++i) {
-        const uint32_t q = kSievePrimes[i];
-        auto inv = [q](uint32_t x) {
-            for (uint32_t y = 1; y < q; ++y)
-                if (x * y % q == 1) return y;
-            return 0u;
-        };
-        uint32_t pk = 1;
-        for (int k = 0; k < 32; ++k) {
-            const uint32_t A = (1 + q * q - pk * pk % q) % q;
-            sk[i * 32 + k] = pk | (A ? inv(A) : 0u) << 8 | inv(2 * pk % q) << 16 | (A ? 1u : 0u) << 24;
-            pk = 2 * pk % q;
-        }
-        for (uint32_t r = 0; r < q; ++r)
-            if (sq[sieve_sqrt_offset(i) + r * r % q] == 0xFF) sq[sieve_sqrt_offset(i) + r * r % q] = (uint8_t)r;
-    }
-    if (e == hipSuccess) e = hipMalloc(&pl->dSieveK, sk.size() * 4);
-    if (e == hipSuccess) e = hipMemcpy(pl->dSieveK, sk.data(), sk.size() * 4, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMalloc(&pl->dSieveSqrt, sq.size());
-    if (e == hipSuccess) e = hipMemcpy(pl->dSieveSqrt, sq.data(), sq.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(pl->dCoprime, t.coprime.data(), kTabWords * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&pl->dSieveK, t.sieveK.size() * 4);
+    if (e == hipSuccess) e = hipMemcpy(pl->dSieveK, t.sieveK.data(), t.sieveK.size() * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&pl->dSieveSqrt, t.sieveSqrt.size());
+    if (e == hipSuccess) e = hipMemcpy(pl->dSieveSqrt, t.sieveSqrt.data(), t.sieveSqrt.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         int rc = hip_fail(e, "search plan upload");
         ffx_search_plan_destroy(pl);
@@ -1343,6 +1419,50 @@ int ffx_search_records(int device, const uint8_t *d_map, uint64_t maxPrimeMapVal
                        first, d_records, d_count);
     FFX_HIP(hipGetLastError());
     FFX_HIP(hipDeviceSynchronize());
+    return FFX_OK;
+}
+
+int ffx_search_records_noalloc(int device, const uint8_t *d_map, uint64_t maxPrimeMapValue, uint32_t sumStart,
+                               uint32_t sumLimit, uint32_t *d_count, ffx_gpu_record *d_records, void *stream)
+{
+    if (!d_map || !d_count || !d_records) return fail(FFX_EINVAL, "search records (no-alloc): null argument");
+    if (device < 0 || device >= kNaMaxDevices) return fail(FFX_ENODEV, "search records (no-alloc): device %d", device);
+    uint8_t *p = g_naScratch[device].load(std::memory_order_acquire);
+    if (!p) return fail(FFX_EINVAL, "search records (no-alloc): call ffx_init(%d) first", device);
+    if ((sumStart & 1) == 0 || sumStart < 5)
+        return fail(FFX_EINVAL, "search records (no-alloc): sumStart must be odd and >= 5");
+    if (sumLimit > FFX_NOALLOC_MAX_SUM)
+        return fail(FFX_ERANGE, "search records (no-alloc): sumLimit %u above the maximum %u of its scratch array",
+                    sumLimit, FFX_NOALLOC_MAX_SUM);
+    if (maxPrimeMapValue < 15) return fail(FFX_EINVAL, "search records (no-alloc): map too small");
+    if (sumLimit < sumStart) return FFX_OK;
+    FFX_HIP(hipSetDevice(device));
+    SearchArgs a = {};
+    a.map = d_map;
+    a.M = maxPrimeMapValue;
+    a.first = sumStart;
+    a.nsums = (sumLimit - sumStart) / 2 + 1;
+    a.slots = reinterpret_cast<uint32_t *>(p + kNaSlotsOff);
+    a.spf = reinterpret_cast<const uint16_t *>(p + kNaSpfOff);
+    a.spfLimit = kNaSpfLimit;
+    a.counter = reinterpret_cast<uint32_t *>(p + kNaCounterOff);
+    a.coprime = reinterpret_cast<const uint32_t *>(p + kNaCoprimeOff);
+    a.sieveK = reinterpret_cast<const uint32_t *>(p + kNaSieveKOff);
+    a.sieveSqrt = p + kNaSqrtOff;
+    a.grab = kGrab;
+    a.eager = kEagerSplits;
+    hipStream_t st = as_stream(stream);
+    // the counter is cleared by a kernel (the array keeps the last call's
+    // contents); every slot in [0, nsums) is written by the search
+    hipLaunchKernelGGL(zero_u32, dim3(1), dim3(64), 0, st, a.counter, 16u);
+    FFX_HIP(hipGetLastError());
+    const uint32_t maxGrid = g_naCus[device] * kBlocksPerCU;
+    const uint32_t need = (a.nsums + kWavesPerBlock - 1) / kWavesPerBlock;
+    hipLaunchKernelGGL(search_sums, dim3(need < maxGrid ? need : maxGrid), dim3(kBlock), 0, st, a);
+    FFX_HIP(hipGetLastError());
+    hipLaunchKernelGGL(slots_to_records, dim3((a.nsums + 255) / 256), dim3(256), 0, st, a.slots, (uint64_t)a.nsums,
+                       sumStart, d_records, d_count);
+    FFX_HIP(hipGetLastError());
     return FFX_OK;
 }
 

@@ -87,6 +87,7 @@ _SIGS = {
     "ffx_compact_async": (_i32, [_vp, _vp, _u64, _u32, _vp, _vp, _vp]),
     "ffx_search_async": (_i32, [_vp, _vp, _u64, _u32, _u32, _vp, _vp, _vp]),
     "ffx_search_records": (_i32, [_i32, _vp, _u64, _u64, _u64, _vp, _vp]),
+    "ffx_search_records_noalloc": (_i32, [_i32, _vp, _u64, _u32, _u32, _vp, _vp, _vp]),
     "ffx_search_intervals_async": (_i32, [_vp, _vp, _u64, _vp, _u32, _u64, _u32, _vp, _vp]),
     "ffx_search_intervals_by_sum_async": (_i32, [_vp, _vp, _u64, _vp, _u32, _u64, _u32, _u32, _vp, _u64, _vp]),
     "ffx_search_plan_set": (_i32, [_vp, _i32, _u32]),

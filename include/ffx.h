@@ -269,6 +269,23 @@ int ffx_search_async(ffx_search_plan *plan, const void *d_map, uint64_t maxPrime
 int ffx_search_records(int device, const uint8_t *d_map, uint64_t maxPrimeMapValue, uint64_t sumStart,
                        uint64_t sumLimit, uint32_t *d_count, ffx_gpu_record *d_records);
 
+/* Largest sum ffx_search_records_noalloc accepts: its scratch array holds
+ * the slots and factor table of odd sums up to here (6.3 MB per device). */
+#define FFX_NOALLOC_MAX_SUM 0x001FFFFFu
+
+/* No counterpart in the reference: ffx_search_records for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  It calls no hipMalloc, hipFree or
+ * allocating routine: its work counter, tables and slots live in a
+ * fixed-size device array (one per device, set up by ffx_init(device), which
+ * must have been called).  Same contract as ffx_search_records (the caller
+ * zeroes d_records and *d_count; solved sums fill their records and add to
+ * *d_count), except that it is asynchronous on `stream` and sumLimit is at
+ * most FFX_NOALLOC_MAX_SUM (FFX_ERANGE above).  Calls that overlap on one
+ * device race on the shared array: order them (one stream, or events). */
+int ffx_search_records_noalloc(int device, const uint8_t *d_map, uint64_t maxPrimeMapValue, uint32_t sumStart,
+                               uint32_t sumLimit, uint32_t *d_count, ffx_gpu_record *d_records, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
