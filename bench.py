@@ -1,29 +1,33 @@
 #!/usr/bin/env python3
-"""Bench: plain search of odd sums [5, 2^20) on one MI355X.
+"""Bench: plain search of odd sums [5, 2^21) on one MI355X.
 
 One step = the hot path over the whole workload with nothing resident but
-the plans' small tables: sieve the 4 GiB odd-only prime bitmap (values up to
-M = 68,719,476,751, ff_seg's geometry for sumLimit 1,048,575), search all
-524,286 odd sums, compact the solutions in sum order.  `value` is odd sums
+the plans' small tables: sieve the 16 GiB odd-only prime bitmap (values up to
+M = 274,877,906,959, ff_seg's geometry for sumLimit 2,097,151), search all
+1,048,574 odd sums, compact the solutions in sum order.  `value` is odd sums
 per second of whole steps.  Kernel times come from HIP events on the stream
 the kernels run on.  Parity is checked on every run: the emitted text of the
 last step hashes to the reference's golden.
 
-Two more legs follow (each its own ff_sieve process, after the plain
-buffers are freed): the batched workload (4096 seeded ranges in one
-graph-captured persistent launch) and the served workload (20000 seeded
-Poisson requests, p50/p99 latency beside sequential plain launches); their
-summaries are attached as "batched" and "served" (the map sieved before the
-first arrival, as the plain launches have it resident), plus
-"served_on_demand" (the map grown while requests arrive).
+Legs after the plain steps (each its own ff_sieve process, after the plain
+buffers are freed, each with its output checked range by range against one
+plain slot array: a mismatch makes the bench exit non-zero):
+  batched           4096 seeded ranges in one graph-captured persistent launch
+  served            20000 seeded Poisson requests at 2000/s, map sieved before
+                    the first arrival, 100 requests spread over the stream
+                    replayed as sequential plain launches beside them
+  served_on_demand  the same stream, map grown while requests arrive
+  served_capacity   the same requests at 12000/s: the device is the bound
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload sums2p20|sums2p21] [--no-cpu] [--no-modes]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload sums2p21|sums2p20] [--no-cpu] [--no-modes]
+  python bench.py --cpu-full      the reference's full CPU runs (ff_seg, 31 and 16 threads, median of 3)
 """
 import argparse
 import ctypes as C
 import hashlib
 import json
 import os
+import platform
 import re
 import subprocess
 import sys
@@ -36,45 +40,31 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 METRIC = "sums searched/s + sieve HBM GB/s; served p50/p99 latency vs plain, 1 MI355X"
 HBM_PEAK_GBS = 8000.0
+LINE_BYTES = 128  # HBM bytes one random single-byte read moves (bin/ff_random_reads + FETCH_SIZE)
 
 WORKLOADS = {
-    # name: (sumLimit, golden leg, description)
-    "sums2p20": (1048575, 1048576, "plain search, odd sums [5, 2^20): one sieve pass (4 GiB map) + one search launch"),
-    "sums2p21": (2097151, 2097152, "plain search, odd sums [5, 2^21): one sieve pass (16 GiB map) + one search launch"),
+    # name: (sumLimit, M, golden leg, description)
+    "sums2p21": (2097151, 274877906959, 2097152,
+                 "plain search, odd sums [5, 2^21): one sieve pass (16 GiB map) + one search launch"),
+    "sums2p20": (1048575, 68719476751, 1048576,
+                 "plain search, odd sums [5, 2^20): one sieve pass (4 GiB map) + one search launch"),
 }
 
-def load_ref_work():
-    p = os.path.join(ROOT, "profiles", "reference_work.json")
-    if os.path.exists(p):
-        return json.load(open(p))
-    return {}
+
+def profile_json(name):
+    p = os.path.join(ROOT, "profiles", name)
+    return json.load(open(p)) if os.path.exists(p) else {}
 
 
-def traffic_for(kernel, workload):
-    p = os.path.join(ROOT, "profiles", "traffic.json")
-    if not os.path.exists(p):
-        return None
-    t = json.load(open(p)).get(workload, {}).get(kernel)
-    return t
-
-
-def probes_for(workload):
-    """Map reads of one search launch, by kind (profiles/search_probes.json,
-    written by `bench.py --count-probes` with the counters build)."""
-    p = os.path.join(ROOT, "profiles", "search_probes.json")
-    if not os.path.exists(p):
-        return None
-    return json.load(open(p)).get(workload)
-
-
-def random_read_ceiling():
-    """The device's rate of random single-byte reads over a 4 GiB buffer
-    (bin/ff_random_reads), the access pattern of the search's map probes."""
+def random_read_ceiling(nbytes):
+    """The device's rate of random single-byte reads over a buffer of the
+    map's size (bin/ff_random_reads), the access pattern of the search's map
+    reads."""
     exe = os.path.join(ROOT, "ff-gpu_amd", "bin", "ff_random_reads")
     if not os.path.exists(exe):
         return None
     try:
-        p = subprocess.run([exe], capture_output=True, timeout=120)
+        p = subprocess.run([exe, str(nbytes)], capture_output=True, timeout=120)
     except subprocess.TimeoutExpired:
         return None
     if p.returncode != 0:
@@ -98,8 +88,7 @@ def count_probes(workload):
     lib.ffx_probe_counters.restype = C.c_int
     lib.ffx_probe_counters.argtypes = [C.c_void_p, C.c_int]
     ffxpy.init(0)
-    sum_limit = WORKLOADS[workload][0]
-    M = 68719476751 if workload == "sums2p20" else 274877906959
+    sum_limit, M = WORKLOADS[workload][:2]
     dmap = ffxpy.sieve_map(M)
     arr = np.zeros(16, dtype=np.uint64)
     ffxpy.check(lib.ffx_probe_counters(arr.ctypes.data, 1))
@@ -111,55 +100,119 @@ def count_probes(workload):
     print(json.dumps({workload: res}))
 
 
-def cpu_baseline(sum_limit, timeout_s):
-    """The reference program (oracle/_ref/ff_seg, built from the reference's
-    source, 31 threads) on the same workload, timed by its own clocks."""
-    exe = os.path.join(ROOT, "oracle", "_ref", "ff_seg")
-    kind = "reference"
-    if not os.path.exists(exe):
-        exe = os.path.join(ROOT, "oracle", "ffref")
-        kind = "port"
-    if not os.path.exists(exe):
-        return None
+def cpu_model():
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        m = re.search(r"Model name:\s*(.+)", out)
+        if m:
+            return m.group(1).strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def host_threads():
+    """Threads the host lets this process use (the GPU box's share, not the
+    machine's core count)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def timed_run(cmd, timeout_s):
     with tempfile.TemporaryDirectory() as d:
         t0 = time.time()
-        try:
-            p = subprocess.run([exe, "5", str(sum_limit)], cwd=d, capture_output=True, timeout=timeout_s)
-        except subprocess.TimeoutExpired:
-            return {"value": None, "unit": "sums/s", "cores": 31, "kind": kind,
-                    "sample": f"timed out after {timeout_s} s"}
+        p = subprocess.run(cmd, cwd=d, capture_output=True, timeout=timeout_s)
         wall = time.time() - t0
     out = p.stdout.decode(errors="replace")
     pt = int(re.search(r"Prime time: (\d+)", out).group(1))
     ft = int(re.search(r"Freudenthal time: (\d+)", out).group(1))
+    return pt * 1e-6, ft * 1e-6, wall
+
+
+def cpu_baseline(workload, timeout_s, sample_k):
+    """The oracle (oracle/ffref, the ff_seg algorithm restated in C) on a
+    bounded, stratified sample of the same workload: the whole sieve of the
+    workload's map (as ff_seg does it) and every sample_k-th odd sum of the
+    search; the search time is scaled by sample_k to the whole range."""
+    exe = os.path.join(ROOT, "oracle", "ffref")
+    if not os.path.exists(exe):
+        return None
+    sum_limit, M = WORKLOADS[workload][:2]
+    threads = min(16, host_threads())
+    try:
+        pt, ft, wall = timed_run([exe, "--threads", str(threads), "--search-sample", str(sample_k), "5",
+                                  str(sum_limit)], timeout_s)
+    except subprocess.TimeoutExpired:
+        return {"value": None, "unit": "sums/s", "cores": threads, "kind": "port",
+                "sample": f"timed out after {timeout_s} s"}
     nodd = (sum_limit - 5) // 2 + 1
-    mapbytes = (2 ** 36 + 16) // 16 if sum_limit == 1048575 else None
+    search_est = ft * sample_k
+    map_bytes = (M >> 4) + 1
     return {
-        "value": nodd / ((pt + ft) * 1e-6),
+        "value": nodd / (pt + search_est),
         "unit": "sums/s",
-        "cores": 31 if kind == "reference" else os.cpu_count(),
-        "kind": kind,
-        "sample": f"full workload 5..{sum_limit} ({nodd} odd sums), one run; ff_seg's own Prime/Freudenthal "
-                  f"clocks: sieve {pt / 1e6:.3f} s, search {ft / 1e6:.3f} s, wall {wall:.2f} s; host has "
-                  f"{os.cpu_count()} CPUs",
-        "sieve_s": pt / 1e6,
-        "search_s": ft / 1e6,
-        "search_sums_per_s": nodd / (ft * 1e-6),
-        "host_cpus": os.cpu_count(),
+        "cores": threads,
+        "kind": "port",
+        "sample": f"oracle/ffref ({threads} threads) on 5..{sum_limit}: the whole sieve ({map_bytes} map bytes, "
+                  f"{pt:.2f} s) and every {sample_k}th odd sum of the search ({ft:.2f} s, x{sample_k} = "
+                  f"{search_est:.1f} s for all {nodd}); wall {wall:.1f} s; host {cpu_model()}",
+        "sieve_s": round(pt, 3),
+        "sieve_gbs": round(map_bytes / pt / 1e9, 3),
+        "search_s_estimated": round(search_est, 2),
+        "search_sums_per_s": nodd / search_est,
+        "host_threads": host_threads(),
+        "cpu_model": cpu_model(),
     }
+
+
+def cpu_full(workload, reps, timeout_s):
+    """The reference program itself (oracle/_ref/ff_seg built from the
+    reference's source) on the whole workload, as the reference's own bench
+    times it (scripts/bench_per_device.sh:56-108): its 31 threads and the
+    host's thread count, median of `reps` runs, its own Prime/Freudenthal
+    clocks."""
+    sum_limit, M = WORKLOADS[workload][:2]
+    nodd = (sum_limit - 5) // 2 + 1
+    res = {"workload": WORKLOADS[workload][3], "odd_sums": nodd, "cpu_model": cpu_model(),
+           "host_threads": host_threads(), "runs": {}}
+    for name, threads in (("ff_seg", 31), ("ff_seg_t16", 16)):
+        exe = os.path.join(ROOT, "oracle", "_ref", name)
+        if not os.path.exists(exe):
+            continue
+        runs = []
+        for r in range(reps):
+            pt, ft, wall = timed_run([exe, "5", str(sum_limit)], timeout_s)
+            runs.append({"prime_s": pt, "freudenthal_s": ft, "wall_s": round(wall, 2)})
+            print(f"[cpu-full] {name} run {r + 1}: prime {pt:.2f} s, search {ft:.2f} s", file=sys.stderr, flush=True)
+        tot = sorted(x["prime_s"] + x["freudenthal_s"] for x in runs)
+        med = tot[len(tot) // 2]
+        res["runs"][name] = {"threads": threads, "runs": runs, "median_s": round(med, 3),
+                             "sums_per_s": nodd / med,
+                             "median_prime_s": sorted(x["prime_s"] for x in runs)[len(runs) // 2],
+                             "median_freudenthal_s": sorted(x["freudenthal_s"] for x in runs)[len(runs) // 2]}
+    print(json.dumps(res))
 
 
 def mode_leg(args, timeout_s):
     """One batched or served run of ff_sieve (its own process, its own
-    device buffers); returns its JSON summary line."""
+    device buffers, its output checked against plain slots); returns its
+    JSON summary line."""
     exe = os.path.join(ROOT, "ff-gpu_amd", "bin", "ff_sieve")
+    print(f"[bench] leg: ff_sieve {' '.join(args)}", file=sys.stderr, flush=True)
     try:
         p = subprocess.run([exe] + args, capture_output=True, timeout=timeout_s)
     except subprocess.TimeoutExpired:
-        return {"error": f"timed out after {timeout_s} s"}
+        return {"error": f"timed out after {timeout_s} s", "parity": "MISMATCH"}
+    try:
+        res = json.loads(p.stdout.decode().strip().splitlines()[-1])
+    except (ValueError, IndexError):
+        return {"error": p.stderr.decode(errors="replace")[-500:], "parity": "MISMATCH"}
     if p.returncode != 0:
-        return {"error": p.stderr.decode(errors="replace")[-500:]}
-    return json.loads(p.stdout.decode().strip().splitlines()[-1])
+        res["error"] = p.stderr.decode(errors="replace")[-500:]
+        res["parity"] = "MISMATCH"
+    return res
 
 
 def calibrate_copy(nbytes):
@@ -178,14 +231,17 @@ def calibrate_copy(nbytes):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="sums2p20", choices=sorted(WORKLOADS))
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="sums2p21", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-timeout", type=float, default=150.0)
+    ap.add_argument("--cpu-timeout", type=float, default=240.0)
+    ap.add_argument("--cpu-sample", type=int, default=32, help="the CPU leg searches every K-th odd sum")
     ap.add_argument("--no-modes", action="store_true", help="skip the batched and served legs")
     ap.add_argument("--count-probes", action="store_true",
                     help="only count the search's map reads by kind (needs make -C ff-gpu_amd counters)")
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="only run the reference's full CPU runs (ff_seg, 31 and 16 threads, median of 3)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only copy a BYTES device buffer three times: the run that scales the HBM byte counters")
     args = ap.parse_args()
@@ -194,6 +250,9 @@ def main():
         return
     if args.count_probes:
         count_probes(args.workload)
+        return
+    if args.cpu_full:
+        cpu_full(args.workload, 3, 900)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the sieve and search are single-device here (each GPU would "
@@ -204,11 +263,10 @@ def main():
     import ffxpy
     from conftest import golden_sha  # noqa: E402  (the golden hash table only)
 
-    sum_limit, leg, desc = WORKLOADS[args.workload]
+    sum_limit, M, leg, desc = WORKLOADS[args.workload]
     lib = ffxpy.load()
     ffxpy.init(0)
     dev = ffxpy.device_info(0)
-    M = 68719476751 if args.workload == "sums2p20" else 274877906959
     map_bytes = (M >> 4) + 1
     nodd = (sum_limit - 5) // 2 + 1
 
@@ -235,6 +293,7 @@ def main():
         ffxpy.check(lib.ffx_event_elapsed_ms(ev[a], ev[b], C.byref(ms)))
         return ms.value
 
+    print(f"[bench] {args.workload}: {args.warmup} warmup + {args.steps} timed steps", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         step()
     ffxpy.check(lib.ffx_stream_sync(stream))
@@ -266,60 +325,70 @@ def main():
 
     sieve_avg = sum(sieve_ms) / len(sieve_ms)
     search_avg = sum(search_ms) / len(search_ms)
-    work = load_ref_work().get(args.workload, {})
-    probes = work.get("probes")
-    # dominant kernel: the search; its algorithmic bytes are the reference's
-    # bitmap probes at one 64-byte line each
-    if probes:
-        ach = probes * 64 / (search_avg * 1e-3) / 1e9
-        traffic = traffic_for("search_sums", args.workload)
-        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "search_sums", "algorithmic": f"{probes} reference-order bitmap probes x 64 B",
-                "traffic_gbs": round(traffic / (search_avg * 1e-3) / 1e9, 1) if traffic else None,
-                "note": "frac above 1: the kernel's residue filter and family-aware split lists skip most of the "
-                        "reference's probes, so it reads far fewer bytes (traffic, measured by rocprofv3 "
-                        "FETCH_SIZE/WRITE_SIZE) than the reference-order work implies; its map reads run near "
-                        "the device's random-read ceiling (random_read_roof) with VALU issue near its peak "
-                        "(VALU wave-instructions per CU per clock in profiles/pmc_plain_2p20.md): both bind"}
-    else:
-        roof = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
-                "traffic": None, "kernel": "search_sums", "algorithmic": "reference probe count missing"}
-    sieve_gbs = map_bytes / (sieve_avg * 1e-3) / 1e9
-    roof_sieve = {"bound": "hbm", "achieved": round(sieve_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                  "frac": round(sieve_gbs / HBM_PEAK_GBS, 4), "traffic": traffic_for("sieve_segments", args.workload),
-                  "kernel": "sieve_segments", "algorithmic": f"{map_bytes} map bytes written once",
-                  "note": "HBM is not what binds the sieve: its strikes are LDS atomics and its issue runs near "
-                          "the VALU peak (VALU wave-instructions per CU per clock in profiles/pmc_plain_2p20.md)"}
-
     for b in (dmap, slots, out, cnt):
         b.close()
     sp.close()
     qp.close()
-    # the search's map reads (counted) against the device's measured rate of
-    # random single-byte reads: the roof for this access pattern, where every
-    # read costs a whole 128-byte line
-    ceiling = random_read_ceiling()
-    pc = probes_for(args.workload)
-    if ceiling and pc:
-        rate = pc["map_reads"] / (search_avg * 1e-3)
-        roof["random_read_roof"] = {"map_reads_per_launch": pc["map_reads"], "achieved_reads_per_s": round(rate, -6),
-                                    "ceiling_reads_per_s": round(ceiling["reads_per_s"], -6),
-                                    "frac": round(rate / ceiling["reads_per_s"], 3),
-                                    "ceiling_from": "bin/ff_random_reads (4 GiB buffer, 2^28 reads, best of 3)"}
-    cpu = None if args.no_cpu else cpu_baseline(sum_limit, args.cpu_timeout)
-    batched = served = served_on_demand = None
+
+    # The search's roofline.  Its HBM work is the map reads it makes (the
+    # counters build counts them per launch), each a random single byte that
+    # moves one whole 128-byte line; the reference-order probes it skips are
+    # reported as work avoided.  Beside HBM: the measured ceiling of random
+    # byte reads over a buffer of the map's size, and VALU issue (pmc run).
+    pc = profile_json("search_probes.json").get(args.workload)
+    ref_work = profile_json("reference_work.json").get(args.workload, {})
+    traffic = profile_json("traffic.json").get(args.workload, {})
+    issue = profile_json("issue.json").get(args.workload, {})
+    ceiling = random_read_ceiling(map_bytes)
+    roof = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
+            "traffic": traffic.get("search_sums"), "kernel": "search_sums"}
+    if pc:
+        reads = pc["map_reads"]
+        ach = reads * LINE_BYTES / (search_avg * 1e-3) / 1e9
+        roof.update({"achieved": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4),
+                     "algorithmic": f"{reads} map reads per launch (counted, profiles/search_probes.json) x "
+                                    f"{LINE_BYTES} B, the line one random single-byte read moves"})
+        if ref_work.get("probes"):
+            roof["work_avoided"] = {"reference_order_probes": ref_work["probes"], "map_reads": reads,
+                                    "ratio": round(ref_work["probes"] / reads, 2)}
+        if ceiling:
+            rate = reads / (search_avg * 1e-3)
+            roof["random_read_roof"] = {"achieved_reads_per_s": round(rate, -6),
+                                        "ceiling_reads_per_s": round(ceiling["reads_per_s"], -6),
+                                        "frac": round(rate / ceiling["reads_per_s"], 3),
+                                        "ceiling_from": f"bin/ff_random_reads over {map_bytes} bytes "
+                                                        "(the map's size), best of 3"}
+    if traffic.get("search_sums"):
+        roof["traffic_gbs"] = round(traffic["search_sums"] / (search_avg * 1e-3) / 1e9, 1)
+    if issue.get("search_sums"):
+        roof["valu_issue_frac"] = issue["search_sums"]["valu_per_cu_clock"]
+    sieve_gbs = map_bytes / (sieve_avg * 1e-3) / 1e9
+    roof_sieve = {"bound": "hbm", "achieved": round(sieve_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                  "frac": round(sieve_gbs / HBM_PEAK_GBS, 4), "traffic": traffic.get("sieve_segments"),
+                  "kernel": "sieve_segments", "algorithmic": f"{map_bytes} map bytes written once"}
+    if issue.get("sieve_segments"):
+        roof_sieve["valu_issue_frac"] = issue["sieve_segments"]["valu_per_cu_clock"]
+    lds = profile_json("lds_floor.json").get(args.workload)
+    if lds:
+        roof_sieve["lds_strike_floor_ms"] = lds["floor_ms"]
+        roof_sieve["frac_of_lds_floor"] = round(lds["floor_ms"] / sieve_avg, 3)
+
+    cpu = None if args.no_cpu else cpu_baseline(args.workload, args.cpu_timeout, args.cpu_sample)
+    cpu_ref = profile_json("cpu_baseline.json").get(args.workload)
+    if cpu and cpu_ref:
+        cpu["reference_full_runs"] = {k: {"threads": v["threads"], "median_s": v["median_s"],
+                                          "sums_per_s": round(v["sums_per_s"], 1)}
+                                      for k, v in cpu_ref.get("runs", {}).items()}
+        cpu["reference_full_runs"]["_from"] = "profiles/cpu_baseline.json (recorded by bench.py --cpu-full)"
+    legs = {}
     if not args.no_modes:
-        # 4096 seeded ranges (starts below 2^22, widths 2^8..2^16) in one
-        # graph-captured persistent launch; the served Poisson stream of
-        # 20000 requests (sumLimit up to 2^23, widths 2^6..2^18, 2000/s)
-        # with 100 of them replayed as sequential plain launches; in served
-        # the map is sieved whole before the first arrival (as the plain
-        # launches have it resident), in served_on_demand it is grown while
-        # requests arrive
-        batched = mode_leg(["--batch", "--opt", "reps=3"], 300)
-        served = mode_leg(["--serve", "--opt", "warm=1", "--opt", "plain_sample=100"], 300)
-        served_on_demand = mode_leg(["--serve"], 300)
+        legs["batched"] = mode_leg(["--batch", "--opt", "reps=3", "--opt", "verify=1"], 400)
+        legs["served"] = mode_leg(["--serve", "--opt", "warm=1", "--opt", "plain_sample=100", "--opt", "verify=1"],
+                                  400)
+        legs["served_on_demand"] = mode_leg(["--serve", "--opt", "verify=1"], 400)
+        legs["served_capacity"] = mode_leg(["--serve", "--opt", "warm=1", "--opt", "rate=12000", "--opt",
+                                            "verify=1"], 400)
+    legs_ok = all(l.get("parity") not in ("MISMATCH", None) and "error" not in l for l in legs.values())
     res = {
         "metric": METRIC,
         "value": nodd / (ms_per_step * 1e-3),
@@ -346,14 +415,12 @@ def main():
         "solutions": c,
         "parity": "sha256 of emitted stdout == reference golden" if parity else "MISMATCH",
         "cpu_baseline": cpu,
-        "batched": batched,
-        "served": served,
-        "served_on_demand": served_on_demand,
     }
+    res.update(legs)
     if cpu and cpu.get("value"):
         res["gpu_over_cpu"] = round(res["value"] / cpu["value"], 1)
     print(json.dumps(res))
-    if not parity:
+    if not parity or not legs_ok:
         sys.exit(1)
 
 

@@ -3,6 +3,7 @@
 #include <algorithm>
 
 #include "emission.h"
+#include "sha256.h"
 
 namespace ffg {
 
@@ -77,6 +78,24 @@ std::vector<std::pair<uint32_t, uint32_t>> subtract(const std::vector<std::pair<
         }
     }
     return out;
+}
+
+const char kNormalizedTimes[] = "Prime time: N \xce\xbcs\nFreudenthal time: N \xce\xbcs\n";
+
+void appendTimes(std::string &out, uint64_t primeUs, uint64_t searchUs)
+{
+    char buf[96];
+    std::snprintf(buf, sizeof buf, "Prime time: %llu \xce\xbcs\nFreudenthal time: %llu \xce\xbcs\n",
+                  (unsigned long long)primeUs, (unsigned long long)searchUs);
+    out += buf;
+}
+
+std::string normalizedDigest(const std::string &body)
+{
+    Sha256 h;
+    h.update(body);
+    h.update(kNormalizedTimes, sizeof kNormalizedTimes - 1);
+    return h.hex();
 }
 
 }  // namespace ffg

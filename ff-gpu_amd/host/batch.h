@@ -47,13 +47,12 @@ IntervalSet unionOf(std::vector<std::pair<uint32_t, uint32_t>> spans);
 std::vector<std::pair<uint32_t, uint32_t>> subtract(const std::vector<std::pair<uint32_t, uint32_t>> &spans,
                                                     const std::vector<std::pair<uint32_t, uint32_t>> &covered);
 
-// Writes one range's stdout.  slotAt(s) returns the slot word of odd sum s
-// (FFX_* encoding).  Returns the number of solution lines.
+// One range's stdout without the two timing lines: its header lines and
+// its solution lines in chunk order, numbered from 1.  slotAt(s) returns the
+// slot word of odd sum s (FFX_* encoding).  Returns the number of solution
+// lines.
 template <class SlotAt>
-uint64_t emitJob(std::string &out, const RangeJob &job, SlotAt slotAt, uint64_t primeUs, uint64_t searchUs);
-
-template <class SlotAt>
-uint64_t emitJob(std::string &out, const RangeJob &job, SlotAt slotAt, uint64_t primeUs, uint64_t searchUs)
+uint64_t emitBody(std::string &out, const RangeJob &job, SlotAt slotAt)
 {
     for (const std::string &h : job.geom.headers) {
         out += h;
@@ -71,10 +70,26 @@ uint64_t emitJob(std::string &out, const RangeJob &job, SlotAt slotAt, uint64_t 
             r.tags = v & ~FFX_SLOT_LOW_MASK;
             out += formatLine(++n, r);
         }
-    char buf[96];
-    std::snprintf(buf, sizeof buf, "Prime time: %llu \xce\xbcs\nFreudenthal time: %llu \xce\xbcs\n",
-                  (unsigned long long)primeUs, (unsigned long long)searchUs);
-    out += buf;
+    return n;
+}
+
+// ff_seg's two timing lines (segmentedSieve.C:922).
+void appendTimes(std::string &out, uint64_t primeUs, uint64_t searchUs);
+
+// The timing lines with their digits replaced by N, as the reference's
+// goldens store every stdout (scripts/verify.sh:37).
+extern const char kNormalizedTimes[];
+
+// SHA-256 (hex) of a range's stdout with normalized timing lines, from its
+// body (emitBody's text).
+std::string normalizedDigest(const std::string &body);
+
+// Body + timing lines: the range's whole stdout.
+template <class SlotAt>
+uint64_t emitJob(std::string &out, const RangeJob &job, SlotAt slotAt, uint64_t primeUs, uint64_t searchUs)
+{
+    const uint64_t n = emitBody(out, job, slotAt);
+    appendTimes(out, primeUs, searchUs);
     return n;
 }
 

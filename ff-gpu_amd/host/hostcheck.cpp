@@ -2,6 +2,7 @@
 // unit tests (no GPU): geometry, headers, partition, budget, config parsing,
 // line formatting and workload generation.  One subcommand per call; output
 // is "key value" lines or one record per line.
+#include <algorithm>
 #include <cinttypes>
 #include <cstdio>
 #include <cstdlib>
@@ -15,6 +16,7 @@
 #include "emission.h"
 #include "geometry.h"
 #include "partition.h"
+#include "sha256.h"
 #include "workload.h"
 
 using namespace ffg;
@@ -186,6 +188,49 @@ int main(int argc, char **argv)
         std::printf("range %u %u\nfatal %d\nany %d\nspan %u %u\n", j.sumStart, j.sumLimit, (int)j.geom.fatal,
                     (int)j.any, j.lo, j.hi);
         for (const SumChunk &x : j.chunks) std::printf("chunk %u %u\n", x.lo, x.hi);
+        return 0;
+    }
+    if (cmd == "sha256") {
+        // SHA-256 of stdin
+        Sha256 h;
+        char buf[1 << 16];
+        size_t n;
+        while ((n = std::fread(buf, 1, sizeof buf, stdin)) > 0) h.update(buf, n);
+        std::printf("%s\n", h.hex().c_str());
+        return 0;
+    }
+    if (cmd == "rangedigest" && argc >= 4) {
+        // stdin: "sum low high" per odd sum (low 0: not emitted).  Prints the
+        // line count and the digest of the range's normalized stdout, built
+        // by the emitter the batched and served modes hash with; the tags
+        // come from trial division here.
+        const RangeJob j = makeJob((uint32_t)u64(argv[2]), (uint32_t)u64(argv[3]), argc >= 5 ? u64(argv[4]) : kSegPrimeMem);
+        std::vector<std::pair<uint32_t, uint32_t>> sol;  // (sum, slot)
+        auto isPrime = [](uint32_t n) {
+            if (n < 2) return false;
+            for (uint32_t d = 2; d * d <= n; ++d)
+                if (n % d == 0) return false;
+            return true;
+        };
+        unsigned a, lo, hi;
+        while (std::scanf("%u %u %u", &a, &lo, &hi) == 3) {
+            if (!lo) continue;
+            uint32_t v = lo;
+            if (isPrime(lo)) v |= FFX_TAG_LOW_PRIME;
+            else if (!(lo & (lo - 1))) v |= FFX_TAG_LOW_POW2;
+            if (isPrime(hi)) v |= FFX_TAG_HIGH_PRIME;
+            else if (!(hi & (hi - 1))) v |= FFX_TAG_HIGH_POW2;
+            sol.push_back({a, v});
+        }
+        std::sort(sol.begin(), sol.end());
+        std::string body;
+        uint64_t lines = 0;
+        if (!j.geom.fatal)
+            lines = emitBody(body, j, [&](uint32_t x) {
+                auto it = std::lower_bound(sol.begin(), sol.end(), std::make_pair(x, 0u));
+                return it != sol.end() && it->first == x ? it->second : 0u;
+            });
+        std::printf("lines %" PRIu64 "\ndigest %s\n", lines, normalizedDigest(body).c_str());
         return 0;
     }
     std::fprintf(stderr, "ff_hostcheck: bad command\n");

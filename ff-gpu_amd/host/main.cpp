@@ -281,6 +281,9 @@ bool modeOptions(const Config &cfg, BatchOptions *b, ServeOptions *sv, AutotuneO
         else if (k == "plain_sample") sv->plainSample = (uint32_t)u;
         else if (k == "warm") sv->warm = u != 0;
         else if (k == "json") sv->jsonOut = v;
+        else if (k == "hashes") b->hashOut = sv->hashOut = v;
+        else if (k == "verify") b->verify = sv->verify = u != 0;
+        else if (k == "slots_out") b->slotsOut = sv->slotsOut = v;
         else if (k == "sum_limit") at->sumLimit = (uint32_t)u;
         else if (k == "out") at->out = v;
         else if (k == "tune_reps") at->reps = (uint32_t)u;

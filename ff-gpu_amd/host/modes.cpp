@@ -25,6 +25,7 @@
 #include <deque>
 #include <fstream>
 #include <sstream>
+#include <climits>
 #include <condition_variable>
 #include <mutex>
 #include <thread>
@@ -33,6 +34,7 @@
 #include "batch.h"
 #include "budget.h"
 #include "ffx.h"
+#include "sha256.h"
 #include "workload.h"
 
 namespace ffg {
@@ -88,6 +90,119 @@ double percentile(std::vector<double> v, double p)
     const size_t i = (size_t)idx;
     const double f = idx - (double)i;
     return i + 1 < v.size() ? v[i] * (1 - f) + v[i + 1] * f : v[i];
+}
+
+// A range's output, hashed: its solution-line count and the SHA-256 of
+// its stdout with normalized timing lines (the fatal text as printed).
+struct Digest {
+    uint64_t lines = 0;
+    std::string hex;
+};
+
+template <class SlotAt>
+Digest digestOf(const RangeJob &j, SlotAt slotAt, std::string &text)
+{
+    text.clear();
+    Digest d;
+    if (j.geom.fatal) {
+        text = fatalText(j);
+        d.hex = sha256Hex(text);
+        return d;
+    }
+    d.lines = emitBody(text, j, slotAt);
+    d.hex = normalizedDigest(text);
+    return d;
+}
+
+// "index sumStart sumLimit lines sha256" per range.
+bool writeDigests(const std::string &path, const std::vector<RangeJob> &jobs, const std::vector<Digest> &d)
+{
+    std::string out;
+    char buf[160];
+    for (size_t i = 0; i < jobs.size(); ++i) {
+        std::snprintf(buf, sizeof buf, "%zu %u %u %llu %s\n", i, jobs[i].sumStart, jobs[i].sumLimit,
+                      (unsigned long long)d[i].lines, d[i].hex.c_str());
+        out += buf;
+    }
+    return writeFile(path, out);
+}
+
+struct PlainCheck {
+    uint64_t checked = 0, mismatched = 0;
+    long firstBad = -1;
+    double searchMs = 0;
+};
+
+// Every range's digest recomputed from ONE plain slot array over the odd
+// sums [5, maxSum] (ffx_search_slots_async on the whole map: the plain path,
+// not the batched or served one), formatted by the same emitter; a range
+// whose digest or line count differs is a mismatch.  slotsOut, when set,
+// receives the plain slot array (u32 per odd sum from 5, little-endian).
+int plainCheck(ffx_search_plan *qp, const void *dMap, uint64_t M, uint32_t maxSum, const std::vector<RangeJob> &jobs,
+               const std::vector<Digest> &got, const std::string &slotsOut, void *stream, PlainCheck *pc)
+{
+    if (maxSum < 5) maxSum = 5;
+    const uint64_t nslots = (maxSum - 5) / 2 + 1;
+    uint32_t *d = (uint32_t *)ffx_alloc(nslots * 4);
+    void *e0 = ffx_event_create(), *e1 = ffx_event_create();
+    AtExit release([&] {
+        ffx_stream_sync(stream);
+        ffx_free(d);
+        ffx_event_destroy(e0);
+        ffx_event_destroy(e1);
+    });
+    if (!d || !e0 || !e1) return fail("plain check buffers", FFX_ENOMEM);
+    int rc;
+    if ((rc = ffx_event_record(e0, stream)) || (rc = ffx_search_slots_async(qp, dMap, M, 5, maxSum, d, stream)) ||
+        (rc = ffx_event_record(e1, stream)) || (rc = ffx_event_sync(e1)))
+        return fail("plain check search", rc);
+    ffx_event_elapsed_ms(e0, e1, &pc->searchMs);
+    std::vector<uint32_t> plain(nslots);
+    if ((rc = ffx_copy_d2h(plain.data(), d, nslots * 4))) return fail("plain check copy", rc);
+    if (!slotsOut.empty() &&
+        !writeFile(slotsOut, std::string(reinterpret_cast<const char *>(plain.data()), nslots * 4))) {
+        std::fprintf(stderr, "[ff_sieve] error: cannot write %s\n", slotsOut.c_str());
+        return 1;
+    }
+    std::atomic<size_t> next{0};
+    std::atomic<uint64_t> bad{0};
+    std::atomic<long> firstBad{LONG_MAX};
+    auto worker = [&] {
+        std::string text;
+        for (size_t i; (i = next.fetch_add(1)) < jobs.size();) {
+            const Digest want = digestOf(jobs[i], [&](uint32_t s) { return plain[(s - 5) >> 1]; }, text);
+            if (want.hex != got[i].hex || want.lines != got[i].lines) {
+                ++bad;
+                long cur = firstBad.load();
+                while ((long)i < cur && !firstBad.compare_exchange_weak(cur, (long)i)) {
+                }
+            }
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 0; t < 8; ++t) th.emplace_back(worker);
+    for (auto &t : th) t.join();
+    pc->checked = jobs.size();
+    pc->mismatched = bad;
+    pc->firstBad = bad ? firstBad.load() : -1;
+    if (bad)
+        std::fprintf(stderr, "[ff_sieve] PARITY MISMATCH: %llu of %zu ranges differ from the plain slots (first: %ld, "
+                             "%u..%u)\n",
+                     (unsigned long long)pc->mismatched, jobs.size(), pc->firstBad, jobs[pc->firstBad].sumStart,
+                     jobs[pc->firstBad].sumLimit);
+    return 0;
+}
+
+std::string parityJson(bool verify, const PlainCheck &pc)
+{
+    if (!verify) return "\"parity\": null";
+    char buf[256];
+    std::snprintf(buf, sizeof buf,
+                  "\"parity\": \"%s\", \"parity_checked\": %llu, \"parity_mismatches\": %llu, "
+                  "\"plain_check_search_ms\": %.3f",
+                  pc.mismatched ? "MISMATCH" : "per-range sha256 == plain slots", (unsigned long long)pc.checked,
+                  (unsigned long long)pc.mismatched, pc.searchMs);
+    return buf;
 }
 
 struct DeviceSetup {
@@ -231,9 +346,11 @@ int runBatch(const Config &cfg, const BatchOptions &opt)
     const double searchMs = *std::min_element(runs.begin(), runs.end());
 
     // per-range stdout, each exactly a plain run's, formatted by 8 host
-    // threads taking ranges in turn
+    // threads taking ranges in turn (hashed for --opt hashes / verify)
     const auto tE = Clock::now();
     if (!opt.outDir.empty()) mkdir(opt.outDir.c_str(), 0755);
+    const bool wantDigest = opt.verify || !opt.hashOut.empty();
+    std::vector<Digest> digests(wantDigest ? jobs.size() : 0);
     std::atomic<size_t> nextRange{0};
     std::atomic<uint64_t> lineCount{0};
     std::atomic<bool> writeFailed{false};
@@ -241,14 +358,14 @@ int runBatch(const Config &cfg, const BatchOptions &opt)
         std::string out;
         uint64_t nl = 0;
         for (size_t i; (i = nextRange.fetch_add(1)) < jobs.size();) {
-            out.clear();
-            if (jobs[i].geom.fatal) out = fatalText(jobs[i]);
-            else
-                nl += emitJob(out, jobs[i], [&](uint32_t s) {
-                    const int64_t k = U.slotOf(s);
-                    return k < 0 ? 0u : slots[(size_t)k];
-                }, (uint64_t)(sieveWall * 1000), (uint64_t)(searchWall * 1000));
+            const Digest d = digestOf(jobs[i], [&](uint32_t s) {
+                const int64_t k = U.slotOf(s);
+                return k < 0 ? 0u : slots[(size_t)k];
+            }, out);
+            nl += d.lines;
+            if (wantDigest) digests[i] = d;
             if (!opt.outDir.empty()) {
+                if (!jobs[i].geom.fatal) appendTimes(out, (uint64_t)(sieveWall * 1000), (uint64_t)(searchWall * 1000));
                 char name[64];
                 std::snprintf(name, sizeof name, "/range_%05zu.txt", i);
                 if (!writeFile(opt.outDir + name, out)) writeFailed = true;
@@ -265,15 +382,27 @@ int runBatch(const Config &cfg, const BatchOptions &opt)
     }
     const uint64_t lines = lineCount;
     const double emitMs = msSince(tE);
+    if (!opt.hashOut.empty() && !writeDigests(opt.hashOut, jobs, digests)) {
+        std::fprintf(stderr, "[ff_sieve] error: cannot write %s\n", opt.hashOut.c_str());
+        return 1;
+    }
+    PlainCheck pc;
+    if (opt.verify && (rc = plainCheck(qp, dMap, M, U.lastSum(), jobs, digests, opt.slotsOut, stream, &pc)))
+        return rc;
+    double searchMean = 0;
+    for (double r : runs) searchMean += r / (double)runs.size();
     std::printf("{\"mode\": \"batch\", \"ranges\": %zu, \"requested_odd_sums\": %llu, \"distinct_odd_sums\": %llu, "
                 "\"intervals\": %zu, \"solution_lines\": %llu, \"map_value\": %llu, \"map_bytes\": %llu, "
-                "\"sieve_ms\": %.3f, \"search_ms\": %.3f, \"emit_ms\": %.3f, \"graph\": %s, "
-                "\"requested_sums_per_s\": %.1f, \"distinct_sums_per_s\": %.1f, \"sieve_gbs\": %.1f}\n",
+                "\"sieve_ms\": %.3f, \"search_ms\": %.3f, \"search_ms_mean\": %.3f, \"search_runs\": %zu, "
+                "\"emit_ms\": %.3f, \"graph\": %s, \"distinct_sums_per_s\": %.1f, "
+                "\"requested_sums_per_s\": %.1f, \"overlap\": %.2f, \"sieve_gbs\": %.1f, %s}\n",
                 jobs.size(), (unsigned long long)requested, (unsigned long long)U.totalSlots, U.iv.size(),
                 (unsigned long long)lines, (unsigned long long)M, (unsigned long long)mapBytes, sieveMs, searchMs,
-                emitMs, graph ? "true" : "false", (double)requested / ((sieveMs + searchMs) * 1e-3),
-                (double)U.totalSlots / (searchMs * 1e-3), (double)mapBytes / (sieveMs * 1e6));
-    return 0;
+                searchMean, runs.size(), emitMs, graph ? "true" : "false", (double)U.totalSlots / (searchMs * 1e-3),
+                (double)requested / ((sieveMs + searchMs) * 1e-3),
+                U.totalSlots ? (double)requested / (double)U.totalSlots : 0.0, (double)mapBytes / (sieveMs * 1e6),
+                parityJson(opt.verify, pc).c_str());
+    return pc.mismatched ? 3 : 0;
 }
 
 // --------------------------------------------------------------------------
@@ -475,21 +604,44 @@ int runServe(const Config &cfg, const ServeOptions &opt)
     const bool writeOut = !opt.outDir.empty();
     if (writeOut) mkdir(opt.outDir.c_str(), 0755);
 
+    // search busy time: per batch, events around its upload and launch,
+    // read against an origin event after the stream ends
+    std::vector<std::pair<void *, void *>> batchEv;
+    void *evOrigin = ffx_event_create();
+    if (!evOrigin) return fail("origin event", FFX_ENOMEM);
+    growEvents.push_back(evOrigin);
+    if ((rc = ffx_event_record(evOrigin, gst))) return fail("origin event", rc);
     const auto t0 = Clock::now();
     auto now = [&] { return std::chrono::duration<double>(Clock::now() - t0).count(); };
+    // a request is done when its stdout is formatted; its digest (when
+    // asked for) is taken after the completion time is recorded
+    const bool wantDigest = opt.verify || !opt.hashOut.empty();
+    std::vector<Digest> digests(wantDigest ? n : 0);
+    std::atomic<bool> writeFailed{false};
     EmitPool pool;
     pool.start(8, [&](size_t id) {
         std::string out;
         uint64_t nl = 0;
         if (jobs[id].geom.fatal) out = fatalText(jobs[id]);
-        else
-            nl = emitJob(out, jobs[id], [&](uint32_t s) { return mirror[(s - base) >> 1]; }, 0, 0);
+        else nl = emitBody(out, jobs[id], [&](uint32_t s) { return mirror[(s - base) >> 1]; });
         if (writeOut) {
+            if (!jobs[id].geom.fatal) appendTimes(out, 0, 0);
             char name[64];
             std::snprintf(name, sizeof name, "/request_%05zu.txt", id);
-            writeFile(opt.outDir + name, out);
+            if (!writeFile(opt.outDir + name, out)) writeFailed = true;
         }
         doneAt[id] = now();
+        if (wantDigest) {
+            Digest d;
+            d.lines = nl;
+            if (jobs[id].geom.fatal) {
+                d.hex = sha256Hex(out);
+            } else {
+                if (writeOut) out.resize(out.size() - (sizeof "Prime time: 0 \xce\xbcs\nFreudenthal time: 0 \xce\xbcs\n" - 1));
+                d.hex = normalizedDigest(out);
+            }
+            digests[id] = std::move(d);
+        }
         std::lock_guard<std::mutex> lk(linesMu);
         lines += nl;
     });
@@ -574,10 +726,17 @@ int runServe(const Config &cfg, const ServeOptions &opt)
             }
             const uint64_t Muse = std::min(M, (mapBytesDone << 4) - 1);
             std::memcpy(hIv[k], iv.data(), iv.size() * sizeof(ffx_interval));
-            if ((rc = ffx_copy_h2d_async(dIv[k], hIv[k], iv.size() * sizeof(ffx_interval), st[k])))
+            void *eb = ffx_event_create(), *ee = ffx_event_create();
+            if (!eb || !ee) return fail("batch events", FFX_ENOMEM);
+            growEvents.push_back(eb);
+            growEvents.push_back(ee);
+            batchEv.push_back({eb, ee});
+            if ((rc = ffx_event_record(eb, st[k])) ||
+                (rc = ffx_copy_h2d_async(dIv[k], hIv[k], iv.size() * sizeof(ffx_interval), st[k])))
                 return fail("interval upload", rc);
             if ((rc = ffx_search_intervals_by_sum_async(qp[k], dMap, Muse, dIv[k], (uint32_t)iv.size(), total,
-                                                        iv.back().last, base, dSlots, nslots, st[k])))
+                                                        iv.back().last, base, dSlots, nslots, st[k])) ||
+                (rc = ffx_event_record(ee, st[k])))
                 return fail("served search", rc);
             for (const ffx_interval &x : iv) {
                 const uint64_t o = (x.first - base) >> 1, cnt = (x.last - x.first) / 2 + 1;
@@ -626,6 +785,10 @@ int runServe(const Config &cfg, const ServeOptions &opt)
     }
     pool.finish();
     const double makespan = now();
+    if (writeFailed) {
+        std::fprintf(stderr, "[ff_sieve] error: cannot write the request files under %s\n", opt.outDir.c_str());
+        return 1;
+    }
     if (!growQ.empty()) {
         if ((rc = ffx_stream_sync(gst))) return fail("map growth", rc);
         mapBytesDone = growQ.back().second;
@@ -634,16 +797,44 @@ int runServe(const Config &cfg, const ServeOptions &opt)
     std::vector<double> lat(n);
     for (size_t i = 0; i < n; ++i) lat[i] = (doneAt[i] - reqs[i].arrivalS) * 1000.0;
     const double p50 = percentile(lat, 0.50), p99 = percentile(lat, 0.99);
+    // device time spent searching: the union of the batches' intervals
+    std::vector<std::pair<double, double>> busyIv;
+    for (const auto &b : batchEv) {
+        double t0ms = 0, t1ms = 0;
+        if ((rc = ffx_event_elapsed_ms(evOrigin, b.first, &t0ms)) || (rc = ffx_event_elapsed_ms(evOrigin, b.second, &t1ms)))
+            return fail("batch event times", rc);
+        busyIv.push_back({t0ms, t1ms});
+    }
+    std::sort(busyIv.begin(), busyIv.end());
+    double busyMs = 0, curLo = 0, curHi = -1;
+    for (const auto &b : busyIv) {
+        if (b.first > curHi) {
+            if (curHi > curLo) busyMs += curHi - curLo;
+            curLo = b.first;
+            curHi = b.second;
+        } else {
+            curHi = std::max(curHi, b.second);
+        }
+    }
+    if (curHi > curLo) busyMs += curHi - curLo;
 
-    // Sequential plain launches of the first plainSample requests at their
-    // arrival times: FIFO, one request at a time, nothing shared between
-    // requests, the map resident at its full size.
-    double pp50 = 0, pp99 = 0, plainRate = 0, plainSpan = 0;
+    // Sequential plain launches of plainSample requests spread evenly over
+    // the stream (every n/plainSample-th), replayed at their arrival times
+    // relative to the first of them: FIFO, one request at a time through the
+    // plain entry point (ffx_search_slots_async), nothing shared between
+    // requests, the map resident at its full size.  The served latencies of
+    // the same requests are reported beside them.
+    double pp50 = 0, pp99 = 0, sp50 = 0, sp99 = 0, plainBusyRate = 0, plainSpan = 0, plainBusyS = 0;
     uint64_t plainSums = 0;
     if (opt.plainSample) {
         const size_t m = std::min<size_t>(opt.plainSample, n);
-        std::vector<double> plat;
-        uint32_t *dS = (uint32_t *)ffx_alloc(nslots * 4);
+        std::vector<size_t> pick;
+        for (size_t j = 0; j < m; ++j) pick.push_back(j * n / m);
+        std::vector<double> plat, slat;
+        uint32_t maxCnt = 1;
+        for (size_t i : pick)
+            if (jobs[i].any && !jobs[i].geom.fatal) maxCnt = std::max<uint32_t>(maxCnt, (jobs[i].hi - jobs[i].lo) / 2 + 1);
+        uint32_t *dS = (uint32_t *)ffx_alloc((uint64_t)maxCnt * 4);
         AtExit freeS([&] {
             ffx_device_sync();
             ffx_free(dS);
@@ -655,57 +846,73 @@ int runServe(const Config &cfg, const ServeOptions &opt)
                 return fail("map", rc);
             mapBytesDone = mapBytesMax;
         }
-        std::vector<uint32_t> hostSlots(nslots);
+        std::vector<uint32_t> hostSlots(maxCnt);
         const auto p0 = Clock::now();
-        const double a0 = reqs[0].arrivalS;
-        for (size_t i = 0; i < m; ++i) {
+        const double a0 = reqs[pick[0]].arrivalS;
+        for (size_t i : pick) {
             const RangeJob &j = jobs[i];
             const double arr = reqs[i].arrivalS - a0;
             const double nowS = std::chrono::duration<double>(Clock::now() - p0).count();
             if (nowS < arr) std::this_thread::sleep_for(std::chrono::duration<double>(arr - nowS));
+            const auto ts = Clock::now();
             if (!j.geom.fatal && j.any) {
-                ffx_interval x;
-                x.first = j.lo;
-                x.last = j.hi;
-                x.slotOffset = 0;
                 const uint64_t cnt = (j.hi - j.lo) / 2 + 1;
-                std::memcpy(hIv[0], &x, sizeof x);
-                if ((rc = ffx_copy_h2d_async(dIv[0], hIv[0], sizeof x, st[0])) ||
-                    (rc = ffx_search_intervals_by_sum_async(qp[0], dMap, M, dIv[0], 1, cnt, j.hi, base, dS, nslots,
-                                                            st[0])))
+                if ((rc = ffx_search_slots_async(qp[0], dMap, M, j.lo, j.hi, dS, st[0])) ||
+                    (rc = ffx_copy_d2h_async(hostSlots.data(), dS, cnt * 4, st[0])) || (rc = ffx_stream_sync(st[0])))
                     return fail("plain search", rc);
-                const uint64_t o = (j.lo - base) >> 1;
-                if ((rc = ffx_copy_d2h_async(hostSlots.data() + o, dS + o, cnt * 4, st[0])) ||
-                    (rc = ffx_stream_sync(st[0])))
-                    return fail("plain copy", rc);
                 plainSums += cnt;
             }
             std::string out;
-            if (!j.geom.fatal) emitJob(out, j, [&](uint32_t s) { return hostSlots[(s - base) >> 1]; }, 0, 0);
+            if (!j.geom.fatal) emitJob(out, j, [&](uint32_t x) { return hostSlots[(x - j.lo) >> 1]; }, 0, 0);
+            plainBusyS += std::chrono::duration<double>(Clock::now() - ts).count();
             plat.push_back((std::chrono::duration<double>(Clock::now() - p0).count() - arr) * 1000.0);
+            slat.push_back(lat[i]);
         }
         plainSpan = std::chrono::duration<double>(Clock::now() - p0).count();
         pp50 = percentile(plat, 0.5);
         pp99 = percentile(plat, 0.99);
-        plainRate = (double)plainSums / plainSpan;
+        sp50 = percentile(slat, 0.5);
+        sp99 = percentile(slat, 0.99);
+        plainBusyRate = plainBusyS > 0 ? (double)plainSums / plainBusyS : 0;
     }
-    char js[1536];
+    if (!opt.hashOut.empty() && !writeDigests(opt.hashOut, jobs, digests)) {
+        std::fprintf(stderr, "[ff_sieve] error: cannot write %s\n", opt.hashOut.c_str());
+        return 1;
+    }
+    PlainCheck pc;
+    if (opt.verify) {
+        if (mapBytesDone < mapBytesMax) {
+            if ((rc = ffx_sieve_map_range_async(sp, dMap, mapBytesDone, mapBytesMax, gst)) ||
+                (rc = ffx_stream_sync(gst)))
+                return fail("map", rc);
+            mapBytesDone = mapBytesMax;
+        }
+        if ((rc = plainCheck(qp[0], dMap, M, maxSum, jobs, digests, opt.slotsOut, st[0], &pc))) return rc;
+    }
+    char js[2560];
     std::snprintf(js, sizeof js,
                   "{\"mode\": \"serve\", \"requests\": %zu, \"rate_per_s\": %.1f, \"requested_odd_sums\": %llu, "
                   "\"searched_odd_sums\": %llu, \"batches\": %llu, \"batch_sums\": %llu, \"streams\": %d, "
-                  "\"makespan_s\": %.3f, \"served_sums_per_s\": %.1f, \"searched_sums_per_s\": %.1f, "
+                  "\"makespan_s\": %.3f, \"offered_sums_per_s\": %.1f, \"search_busy_s\": %.3f, "
+                  "\"searched_sums_per_busy_s\": %.1f, \"searched_sums_per_s\": %.1f, "
                   "\"p50_ms\": %.3f, \"p99_ms\": %.3f, \"map_value\": %llu, \"map_bytes\": %llu, "
                   "\"map_growths\": %llu, \"solution_lines\": %llu, \"plain_sample\": %u, \"plain_span_s\": %.3f, "
-                  "\"plain_sums_per_s\": %.1f, \"plain_p50_ms\": %.3f, \"plain_p99_ms\": %.3f, \"warm\": %s, "
-                  "\"map_prebuilt_s\": %.3f}\n",
+                  "\"plain_busy_s\": %.3f, \"plain_sums_per_busy_s\": %.1f, \"plain_p50_ms\": %.3f, "
+                  "\"plain_p99_ms\": %.3f, \"served_sample_p50_ms\": %.3f, \"served_sample_p99_ms\": %.3f, "
+                  "\"warm\": %s, \"map_prebuilt_s\": %.3f, %s}\n",
                   n, opt.rate, (unsigned long long)requested, (unsigned long long)searchedSlots,
                   (unsigned long long)batches, (unsigned long long)batchSums, S, makespan,
-                  (double)requested / makespan, (double)searchedSlots / makespan, p50, p99, (unsigned long long)M,
-                  (unsigned long long)mapBytesMax, (unsigned long long)growths, (unsigned long long)lines,
-                  opt.plainSample, plainSpan, plainRate, pp50, pp99, opt.warm ? "true" : "false", mapBuildS);
+                  (double)requested / makespan, busyMs * 1e-3,
+                  busyMs > 0 ? (double)searchedSlots / (busyMs * 1e-3) : 0.0, (double)searchedSlots / makespan, p50,
+                  p99, (unsigned long long)M, (unsigned long long)mapBytesMax, (unsigned long long)growths,
+                  (unsigned long long)lines, opt.plainSample, plainSpan, plainBusyS, plainBusyRate, pp50, pp99, sp50,
+                  sp99, opt.warm ? "true" : "false", mapBuildS, parityJson(opt.verify, pc).c_str());
     std::fputs(js, stdout);
-    if (!opt.jsonOut.empty()) writeFile(opt.jsonOut, js);
-    return 0;
+    if (!opt.jsonOut.empty() && !writeFile(opt.jsonOut, js)) {
+        std::fprintf(stderr, "[ff_sieve] error: cannot write %s\n", opt.jsonOut.c_str());
+        return 1;
+    }
+    return pc.mismatched ? 3 : 0;
 }
 
 // --------------------------------------------------------------------------

@@ -18,6 +18,9 @@ struct BatchOptions {
     std::string outDir;          // one file per range when set
     uint32_t reps = 1;           // graph replays timed
     bool useGraph = true;
+    std::string hashOut;         // per-range digests of the normalized stdout
+    bool verify = false;         // check every range against one plain slot array
+    std::string slotsOut;        // verify: the plain slot array (u32 per odd sum from 5)
 };
 
 struct ServeOptions {
@@ -28,10 +31,14 @@ struct ServeOptions {
     double rate = 2000.0;        // requests per second (Poisson)
     uint64_t batchSums = 65536;   // odd sums per persistent launch (the oldest queued first)
     uint32_t streams = 2;        // batches in flight
-    uint32_t plainSample = 0;    // also time this many requests as sequential plain launches
+    uint32_t plainSample = 0;    // also time this many requests, spread evenly over the stream, as
+                                 // sequential plain launches
     bool warm = false;           // sieve the whole map before the first arrival (else grown on demand)
     std::string outDir;
     std::string jsonOut;         // summary line
+    std::string hashOut;         // per-request digests of the normalized stdout
+    bool verify = false;         // check every request against one plain slot array
+    std::string slotsOut;        // verify: the plain slot array (u32 per odd sum from 5)
 };
 
 struct AutotuneOptions {

@@ -3,8 +3,11 @@
 Every range of a batch and every request of a served stream must print
 exactly what a plain `ff_sieve <sumStart> <sumLimit>` (and ff_seg) prints for
 it alone: checked against the oracle (oracle/ffref, ff_seg-identical stdout)
-on small sums, and against plain GPU runs on the full-size batched workload
-(4096 seeded ranges, starts below 2^22, widths 2^8..2^16)."""
+on small sums, and at full size (the 4096-range batch, the 20000-request
+served stream) range by range against one plain slot array, itself checked
+against the oracle's answers for a seeded sample of sums above 2^21."""
+import gzip
+import hashlib
 import json
 import os
 import random
@@ -12,12 +15,11 @@ import subprocess
 import sys
 import tempfile
 
+import numpy as np
 import pytest
 
-import hashlib
-
 import ffxpy
-from conftest import golden_sha, normalize, run_ffref
+from conftest import GOLDEN, golden_sha, normalize, run_ffref
 
 pytestmark = pytest.mark.gpu
 
@@ -71,24 +73,61 @@ def test_batch_ranges_match_oracle(graph):
         assert s["solution_lines"] == lines
 
 
-def test_batch_c4_ranges_match_plain_runs():
-    # the full batched workload; a sample of its ranges (the largest sums
-    # among them) against plain single-range GPU runs
+def fixture_beyond_2p21():
+    """The oracle's F(s) for 6000 seeded sums in [2^21, 2^23]
+    (tests/golden/make_solve_fixtures.py): {sum: low}, 0 = not emitted."""
+    rows = [l.split() for l in gzip.open(os.path.join(GOLDEN, "solve_beyond_2p21.txt.gz"), "rt") if l.strip()]
+    return {int(s): int(lo) for s, lo, _ in rows}
+
+
+def check_plain_slots_against_oracle(path, max_sum):
+    """The plain slot array a verified run wrote (u32 per odd sum from 5 to
+    the largest sum any range asks for, at most max_sum) against the
+    oracle's sample: the check of every range rests on it."""
+    slots = np.fromfile(path, dtype="<u4")
+    top = 5 + 2 * (len(slots) - 1)
+    assert max_sum - 2048 < top <= max_sum, (top, max_sum)
+    max_sum = top
+    fx = {s: lo for s, lo in fixture_beyond_2p21().items() if s <= max_sum}
+    assert len(fx) > 1000
+    got = {s: int(slots[(s - 5) // 2]) & ffxpy.SLOT_LOW_MASK for s in fx}
+    bad = [s for s in fx if got[s] != fx[s]]
+    assert not bad, bad[:10]
+    return len(fx), sum(1 for v in fx.values() if v)
+
+
+def test_batch_c4_every_range_matches_plain_slots():
+    # the full batched workload (4096 seeded ranges, one graph-captured
+    # launch): every range's stdout digest against the same range formatted
+    # from one plain slot array over [5, max sum], which in turn matches the
+    # oracle's answers for the seeded sums above 2^21
     with tempfile.TemporaryDirectory() as d:
-        s = summary(ff_sieve("--batch", "--opt", f"out_dir={d}", timeout=600))
+        slots = os.path.join(d, "plain_slots.u32")
+        s = summary(ff_sieve("--batch", "--opt", "verify=1", "--opt", f"slots_out={slots}", "--opt",
+                             f"hashes={os.path.join(d, 'digests.txt')}", timeout=600))
         assert s["ranges"] == 4096 and s["graph"] is True
-        ranges = [tuple(map(int, l.split())) for l in subprocess.run(
-            [os.path.join(os.path.dirname(ffxpy.FF_SIEVE), "ff_hostcheck"), "batch", "20261203", "4096"],
-            capture_output=True, text=True, check=True).stdout.splitlines()]
-        assert len(ranges) == 4096
-        order = sorted(range(4096), key=lambda i: -ranges[i][1])
-        pick = order[:3] + random.Random(5).sample(order[3:], 5)
-        for i in pick:
-            a, b = ranges[i]
-            plain = ff_sieve(a, b)
-            assert plain.returncode == 0
-            got = normalize(open(os.path.join(d, f"range_{i:05d}.txt")).read())
-            assert got == normalize(plain.stdout.decode()), (i, a, b)
+        assert s["parity_checked"] == 4096 and s["parity_mismatches"] == 0, s
+        rows = open(os.path.join(d, "digests.txt")).read().splitlines()
+        assert len(rows) == 4096 and sum(int(r.split()[3]) for r in rows) == s["solution_lines"]
+        max_sum = max(int(r.split()[2]) for r in rows)
+        n, emitted = check_plain_slots_against_oracle(slots, max_sum if max_sum % 2 else max_sum - 1)
+        print(f"[case] C4: 4096 ranges match the plain slots; {n} oracle sums ({emitted} emitted) match", flush=True)
+
+
+def test_serve_c5_every_request_matches_plain_slots():
+    # the served stream as configured (seed 20261204: 20000 Poisson requests
+    # at 2000/s, sumLimit up to 2^23, widths 2^6..2^18, map grown on demand):
+    # every request's stdout digest, taken as it completes in the mix,
+    # against the request formatted from one plain slot array over [5, 2^23]
+    # (itself checked against the oracle's sample)
+    with tempfile.TemporaryDirectory() as d:
+        slots = os.path.join(d, "plain_slots.u32")
+        s = summary(ff_sieve("--serve", "--opt", "verify=1", "--opt", f"slots_out={slots}", timeout=900))
+        assert s["requests"] == 20000 and s["map_growths"] > 1
+        assert s["parity_checked"] == 20000 and s["parity_mismatches"] == 0, s
+        n, emitted = check_plain_slots_against_oracle(slots, (1 << 23) - 1)
+        print(f"[case] C5: 20000 requests match the plain slots; {n} oracle sums ({emitted} emitted) match",
+              flush=True)
 
 
 def test_serve_requests_match_oracle_alone():
@@ -115,7 +154,7 @@ def test_serve_mix_matches_plain_on_large_sums():
     # against a plain run
     count = 24
     with tempfile.TemporaryDirectory() as d:
-        s = summary(ff_sieve("--map-cap", 2 ** 37, "--serve", "--opt", f"count={count}", "--opt", "seed=9",
+        s = summary(ff_sieve("--map-cap", 2 ** 37, "--serve", "--opt", "verify=1", "--opt", f"count={count}", "--opt", "seed=9",
                              "--opt", "wlo=6", "--opt", "whi=12", "--opt", "rate=500", "--opt", f"out_dir={d}",
                              timeout=600))
         assert s["requests"] == count

@@ -252,3 +252,31 @@ def test_serve_admission_against_the_budget():
     assert hc("servemap", 512 * GiB, 16 * MiB, 2 ** 23 - 1, 2 ** 34 - 1).stdout.strip() == str(2 ** 34 - 1)
     # small sums: the uncapped bound of 1048575 is ff_seg's own M for that leg
     assert hc("servemap", 512 * GiB, 16 * MiB, 1048575, 0).stdout.strip() == "68719476751"
+
+
+def test_sha256_matches_hashlib():
+    # the digest the batched and served modes write per range
+    import hashlib
+    rng = random.Random(41)
+    for n in [0, 1, 3, 55, 56, 63, 64, 65, 119, 120, 4095, 100003]:
+        data = bytes(rng.getrandbits(8) for _ in range(n))
+        p = subprocess.run([HC, "sha256"], input=data, capture_output=True)
+        assert p.returncode == 0
+        assert p.stdout.decode().strip() == hashlib.sha256(data).hexdigest(), n
+
+
+@pytest.mark.parametrize("a,b", [(1001, 3000), (4, 6), (5, 5), (100, 50), (60001, 70000), (5, 2627),
+                                 (131001, 131071), (262001, 262100)])
+def test_range_digest_matches_oracle_stdout(ffref, a, b):
+    # the emitter and digest of the batched and served modes, fed the
+    # oracle's per-sum answers, hash to the oracle's own ff_seg stdout
+    import hashlib
+    from conftest import normalize
+    want = hashlib.sha256(normalize(run_ffref([a, b]).stdout.decode()).encode()).hexdigest()
+    first, last = max(5, a | 1), max(b, max(5, a | 1))
+    sums = "\n".join(str(s) for s in range(first, last + 2, 2)).encode()
+    solved = run_ffref(["--solve"], stdin=sums).stdout
+    p = subprocess.run([HC, "rangedigest", str(a), str(b)], input=solved, capture_output=True)
+    assert p.returncode == 0, p.stderr
+    got = dict(l.split(" ", 1) for l in p.stdout.decode().splitlines())
+    assert got["digest"] == want, (a, b)

@@ -166,3 +166,17 @@ def test_oracle_solve_matches_ff_seg_beyond_the_goldens(ffref):
         got = {int(s): int(lo) for s, lo, _ in (l.split() for l in p.stdout.decode().split("\n") if l) if lo != "0"}
         assert got == want, c["args"]
         assert want  # every case prints solutions
+
+
+def test_solve_fixture_beyond_2p21_is_the_oracles(ffref):
+    # tests/golden/solve_beyond_2p21.txt.gz (make_solve_fixtures.py): a
+    # seeded slice of it recomputed by the current oracle
+    import gzip
+    import random
+    rows = [l.split() for l in gzip.open(os.path.join(GOLDEN, "solve_beyond_2p21.txt.gz"), "rt") if l.strip()]
+    assert len(rows) == 6000
+    assert sum(1 for r in rows if r[1] != "0") > 300
+    pick = random.Random(9).sample(rows, 96)
+    p = run_ffref(["--solve"], stdin="\n".join(r[0] for r in pick).encode())
+    got = [l.split() for l in p.stdout.decode().splitlines() if l]
+    assert got == pick
