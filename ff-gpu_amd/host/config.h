@@ -1,13 +1,13 @@
-// ff_sieve configuration: CLI flags and FF_* environment variables,
-// precedence CLI > env > default (source/config.h:15-85, config.cpp:156-445).
+// ff_sieve configuration: command-line flags and FF_* environment variables,
+// the command line winning over the environment and both over the defaults.
 //
-// Kept knobs are those of one gfx950 device: VRAM fraction/budget, scratch,
-// slab size, host tier (validated; the map never leaves HBM unless dumped),
-// --dump-map, --list-devices.  Vendor selection (--devices, FF_DISABLE_DEVICE,
-// FF_PULL_WEIGHTS) is parsed with the reference's error behaviour; only
-// "amd" can match.  New knobs: --primemem (ff_seg's PRIMEMEM, the header and
-// map cap), --map-cap (largest value the device bitmap holds; Miller-Rabin
-// answers above it).
+// The flag and variable names and the syntax of their values are those of
+// the reference's ff_sieve (source/config.h:15-85), so its command lines keep
+// working; what they configure is one gfx950 device.  Knobs of the
+// reference's multi-vendor scheduler are refused with a message instead of
+// being stored.  New knobs: --primemem (ff_seg's PRIMEMEM, the header and map
+// cap), --map-cap (largest value the device bitmap holds; Miller-Rabin
+// answers above it), kernel tuning, and the batched/served/autotune modes.
 #ifndef FFG_CONFIG_H
 #define FFG_CONFIG_H
 
@@ -26,15 +26,11 @@ struct Config {
     bool hasScratch = false;                        // --scratch / FF_SCRATCH
     uint64_t scratchBytes = 0;
     uint64_t slabSizeBytes = 1ull << 30;            // --slab-size / FF_SLAB_SIZE
-    bool hostTierAuto = false;                      // --host-tier-cap / FF_HOST_TIER_CAP
+    bool hostTierAuto = false;                      // --host-tier-cap / FF_HOST_TIER_CAP ("auto" or a size)
     uint64_t hostTierCapBytes = 0;
     bool noHostTier = false;                        // --no-host-tier
     bool listDevices = false;                       // --list-devices
-    std::string deviceFilter;                       // --devices amd|nvidia
-    std::string disableVendor;                      // FF_DISABLE_DEVICE
-    std::map<std::string, double> pullWeights;      // FF_PULL_WEIGHTS
     std::string dumpMapFile;                        // --dump-map <file>
-    bool gpuSearch = true;                          // --gpu-search (the default here)
     uint64_t primeMem = 85ull << 30;                // --primemem
     uint64_t mapCap = 0;                            // --map-cap (0 = ff_seg's M)
     int device = 0;                                 // --device
@@ -47,17 +43,22 @@ struct Config {
     std::vector<std::string> modeArgs;              // key=value options of the mode
 };
 
-// Reads the key=value lines (or a flat JSON object of numbers) of a tuning
-// file into cfg's tuning fields.  Returns 0, or -1 with a message.
-int loadTuneFile(Config *cfg, const std::string &path);
+// Environment, then argv (non-option words go to *positionals), then the
+// checks and derived values (tune file, host tier).  0, or -1 after a
+// message on stderr.
+int loadConfig(int argc, char **argv, Config *cfg, std::vector<std::string> *positionals);
 
-bool parseSize(const std::string &s, uint64_t *out);
-bool parseFraction(const std::string &s, double *out);
-bool parseDeviceFractionSpec(const std::string &spec, std::map<std::string, double> *out);
-int loadEnv(Config *cfg);
-int parseArgs(int argc, char **argv, Config *cfg, std::vector<std::string> *positionals);
-int validateConfig(Config *cfg);
-bool slabSizeValueAligned(uint64_t slabSizeBytes);
+// Value syntax shared with the reference's ff_sieve: a byte count with an
+// optional K/M/G unit (KiB = KB = K = 1024 and so on, case-insensitive,
+// fractional counts allowed: "1.5GiB"); a non-negative decimal fraction; a
+// comma-separated list of key=fraction.
+bool readByteSize(const std::string &text, uint64_t *out);
+bool readFraction(const std::string &text, double *out);
+bool readFractionMap(const std::string &text, std::map<std::string, double> *out);
+
+// Reads the key=value lines (or a flat JSON object of numbers) of a tuning
+// file into cfg's tuning fields that no flag set.  0, or -1 with a message.
+int loadTuneFile(Config *cfg, const std::string &path);
 
 }  // namespace ffg
 

@@ -85,19 +85,19 @@ int main(int argc, char **argv)
     }
     if (cmd == "size" && argc >= 3) {
         uint64_t v = 0;
-        if (!parseSize(argv[2], &v)) return 1;
+        if (!readByteSize(argv[2], &v)) return 1;
         std::printf("%" PRIu64 "\n", v);
         return 0;
     }
     if (cmd == "fraction" && argc >= 3) {
         double v = 0;
-        if (!parseFraction(argv[2], &v)) return 1;
+        if (!readFraction(argv[2], &v)) return 1;
         std::printf("%.17g\n", v);
         return 0;
     }
     if (cmd == "devfrac" && argc >= 3) {
         std::map<std::string, double> m;
-        if (!parseDeviceFractionSpec(argv[2], &m)) return 1;
+        if (!readFractionMap(argv[2], &m)) return 1;
         for (const auto &kv : m) std::printf("%s %.17g\n", kv.first.c_str(), kv.second);
         return 0;
     }
@@ -108,7 +108,7 @@ int main(int argc, char **argv)
         std::vector<char *> av;
         av.push_back(argv[0]);
         for (int i = 2; i < argc; ++i) av.push_back(argv[i]);
-        if (loadEnv(&cfg) || parseArgs((int)av.size(), av.data(), &cfg, &pos) || validateConfig(&cfg)) return 1;
+        if (loadConfig((int)av.size(), av.data(), &cfg, &pos)) return 1;
         std::printf("fraction %.17g\nbudgetCap %" PRIu64 "\nscratch %" PRIu64 "\nslab %" PRIu64 "\nhostTier %" PRIu64
                     "\nprimemem %" PRIu64 "\nmapCap %" PRIu64 "\npositionals %zu\n",
                     cfg.globalFraction, cfg.hasBudgetCap ? cfg.budgetCapBytes : 0,

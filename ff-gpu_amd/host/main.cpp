@@ -100,16 +100,6 @@ int runLeg(const Config &cfg, const std::vector<std::string> &pos)
         std::fprintf(stderr, "[ff_sieve] error: sumLimit %u above the supported %u\n", sumLimit, FFX_MAX_SUM);
         return 1;
     }
-    if (!cfg.disableVendor.empty() && cfg.disableVendor == "amd") {
-        std::fprintf(stderr, "[ff_sieve] ERROR: FF_DISABLE_DEVICE=amd leaves no devices (enumerated: amd)\n");
-        return 1;
-    }
-    if (!cfg.deviceFilter.empty() && cfg.deviceFilter != "amd") {
-        std::fprintf(stderr, "[ff_sieve] ERROR: no device matches --devices=%s (enumerated: amd)\n",
-                     cfg.deviceFilter.c_str());
-        return 1;
-    }
-
     auto tEnum = std::chrono::steady_clock::now();
     int rc = ffx_init(cfg.device);
     if (rc) return err("device init", rc);
@@ -304,7 +294,7 @@ int main(int argc, char **argv)
     {
         Config cfg;
         std::vector<std::string> pos;
-        if (loadEnv(&cfg) || parseArgs(argc, argv, &cfg, &pos) || validateConfig(&cfg)) ret = 1;
+        if (loadConfig(argc, argv, &cfg, &pos)) ret = 1;
         else if (cfg.listDevices) ret = listDevices();
         else if (cfg.mode == "batch" || cfg.mode == "serve" || cfg.mode == "autotune") {
             BatchOptions b;
