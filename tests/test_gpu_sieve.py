@@ -137,3 +137,33 @@ def test_popcount_is_pi(k, pi):
         n = min(step, nb - off)
         total += int(np.bitwise_count(buf.to_host(n, offset=off)).sum())
     assert total == pi + small_primes_tail(2 ** k + 1, M) - 1
+
+
+def map_hashes():
+    """sumLimit -> (map bytes, sha256) of the reference program's own
+    --dump-map (tests/golden/make_map_hashes.py)."""
+    out = {}
+    with open(os.path.join(os.path.dirname(__file__), "golden", "map_sha256.txt")) as f:
+        for line in f:
+            L, n, h = line.split()
+            out[int(L)] = (int(n), h)
+    return out
+
+
+@pytest.mark.parametrize("sum_limit", [1048575, 2097151])
+def test_map_sha256_matches_reference_dump(sum_limit):
+    # the whole 4 GiB / 16 GiB device map of odd sums [5, 2^20) and
+    # [5, 2^21) hashes to the reference's own dump of it
+    # (scripts/verify.sh:156-173 makes the same check)
+    import hashlib
+    n, want = map_hashes()[sum_limit]
+    M = {1048575: 68719476751, 2097151: 274877906959}[sum_limit]
+    assert ffxpy.map_bytes_for(M) == n
+    say("map sha256", sum_limit)
+    buf = ffxpy.sieve_map(M)
+    h = hashlib.sha256()
+    step = 1 << 29
+    for off in range(0, n, step):
+        h.update(buf.to_host(min(step, n - off), offset=off))
+    buf.close()
+    assert h.hexdigest() == want
