@@ -10,7 +10,7 @@
 //
 // Kernel design (gfx950, wave64, 160 KiB LDS per CU):
 //  * One workgroup of 1024 threads owns one map segment at a time in LDS
-//    (32 KiB = 2^18 odd values below maps of 2^38 values, 64 KiB from there)
+//    (32 KiB = 2^18 odd values below maps of 2^38 values, 112 KiB from there)
 //    and sieves a contiguous run of segments.
 //  * Bits are kept "struck" = 1 in LDS, in natural order (bit b of word w is
 //    odd index 32w+b); the write-out inverts them and reorders each byte to
@@ -36,12 +36,14 @@ namespace {
 // Defaults measured on MI355X (ff_sieve --autotune and direct timing):
 // maps whose sieving primes stay below 2^19 (M < 2^38) are fastest with
 // 32 KiB segments and four 1024-thread workgroups per CU (two resident with
-// the 40 KiB of patterns): 4 GiB map 9.45 ms against 10.15 with 64 KiB.
+// the 40 KiB of patterns): 4 GiB map 9.0 ms against 9.4 with 112 KiB.
 // Larger maps, where most primes strike a segment at most a few times and
 // each segment pays every prime's first-multiple computation, are faster
-// with 64 KiB segments, two per CU: 16 GiB map 43.4 against 46.1 ms.
+// with the largest segment the LDS holds beside the patterns, one
+// workgroup per CU: 16 GiB map 39.5 ms with 112 KiB against 40.1 (96 KiB),
+// 41.2 (80 KiB) and 41.8 (64 KiB, two per CU).
 constexpr uint32_t kSmallMapSegBytes = 32768, kSmallMapBlocksPerCU = 4;
-constexpr uint32_t kLargeMapSegBytes = 65536, kLargeMapBlocksPerCU = 2;
+constexpr uint32_t kLargeMapSegBytes = 114688, kLargeMapBlocksPerCU = 1;
 constexpr uint64_t kLargeMapFrom = 1ull << 38;
 constexpr int kThreads = 1024;
 constexpr int kWaves = kThreads / 64;
@@ -65,6 +67,25 @@ constexpr uint32_t pattern_words()
     return s;
 }
 constexpr uint32_t kPatWords = pattern_words();  // 10238 words, 40 KiB
+
+// Odd primes in (kLargestPatternPrime, x): the cooperative primes of a
+// segment of 64x odd values, counted at compile time to size their table.
+constexpr uint32_t coop_primes_below(uint32_t x)
+{
+    uint32_t n = 0;
+    for (uint32_t c = kLargestPatternPrime + 2; c < x; c += 2) {
+        bool prime = true;
+        for (uint32_t d = 3; d * d <= c; d += 2)
+            if (c % d == 0) {
+                prime = false;
+                break;
+            }
+        n += prime;
+    }
+    return n;
+}
+// table entries for a segment of segOdds odd values, rounded up to 64
+constexpr uint32_t coop_capacity(uint32_t segOdds) { return (coop_primes_below(segOdds / 64) + 63) & ~63u; }
 
 struct SieveArgs {
     uint8_t *dst;          // receives map bytes [byteLo, byteHi) at dst[byte - dstBase]
@@ -128,24 +149,25 @@ __device__ __forceinline__ uint32_t to_map_order(uint32_t v)
 // each segment with first_strike.  kHoldPrimes keeps the primes themselves
 // in registers too; otherwise they are re-read each segment (L2-resident),
 // which halves the registers per carried prime.  32 KiB segments run eight
-// waves per SIMD (64 registers per lane), 64 KiB segments four (128).
-// Measured on MI355X (4 GiB / 16 GiB / 70 GB maps): 32 KiB with 12 re-read
-// carries 9.45 ms (8: 9.7, none: 10.1); 64 KiB with 32 held 43.4 ms on
+// waves per SIMD (64 registers per lane), the large segments four (128).
+// Measured on MI355X with 64 KiB large segments (4 GiB / 16 GiB / 70 GB
+// maps): 32 KiB with 12 re-read carries 9.45 ms (8: 9.7, none: 10.1);
+// 64 KiB with 32 held 43.4 ms on
 // 16 GiB (64 re-read: 44.9) but 214 ms on 70 GB, where 64 re-read take 202.
 #ifndef FFX_FILL_UNROLL
 #define FFX_FILL_UNROLL 4
 #endif
 constexpr int kCarrySmall = 12;      // 32 KiB segments, primes re-read
-constexpr int kCarryHeld = 32;       // 64 KiB segments, primes held
-constexpr int kCarryReread = 64;     // 64 KiB segments, primes re-read
-constexpr uint32_t kHeldUpTo = 48 * 1024;  // lane primes up to which 64 KiB holds them
+constexpr int kCarryHeld = 32;       // large segments, primes held
+constexpr int kCarryReread = 64;     // large segments, primes re-read
+constexpr uint32_t kHeldUpTo = 48 * 1024;  // lane primes up to which the large kernel holds them
 
 template <uint32_t kSegBytes, int kCarry, bool kHoldPrimes>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kSegBytes == 32768 ? 8 : 4))) void sieve_segments(SieveArgs a)
 {
     constexpr uint32_t kSegWords = kSegBytes / 4;
     constexpr uint32_t kSegOdds = kSegBytes * 8;  // odd values per segment
-    constexpr uint32_t kCoopMax = kSegOdds / 64 / 4;  // > the number of odd primes below kSegOdds/64
+    constexpr uint32_t kCoopMax = coop_capacity(kSegOdds);  // the odd primes 67..kSegOdds/64
     __shared__ uint32_t seg[kSegWords];
     __shared__ uint32_t pat[kPatWords];
     __shared__ uint32_t coopNext[kCoopMax];  // first strike of each cooperative prime, from the segment's start
@@ -165,7 +187,11 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kSegBy
     // Each workgroup sieves a contiguous run of segments, so a cooperative
     // prime's first strike in the next segment follows from this one's
     // (kept in LDS) instead of a 64-bit division per segment.
-    const uint64_t perBlock = (a.segCount + gridDim.x - 1) / gridDim.x;
+    // (uniform: moved to scalar registers, so the 64-bit division's result
+    // does not hold vector registers through the run)
+    uint64_t perBlock = (a.segCount + gridDim.x - 1) / gridDim.x;
+    perBlock = (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(perBlock >> 32)) << 32 |
+               __builtin_amdgcn_readfirstlane((uint32_t)perBlock);
     const uint64_t runLo = (uint64_t)blockIdx.x * perBlock;
     const uint64_t runHi = runLo + perBlock < a.segCount ? runLo + perBlock : a.segCount;
     {
@@ -259,7 +285,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kSegBy
                         uint32_t ad = (j >> 5) << 2;
                         if (kSegBytes == 32768) {
                             // two strikes per step while both are below the
-                            // end (4 GiB map 9.08 -> 8.93 ms; the 64 KiB
+                            // end (4 GiB map 9.08 -> 8.93 ms; the large-segment
                             // kernels measured slower with it)
                             const uint32_t endPair = endBytes > stepBytes ? endBytes - stepBytes : 0u;
                             for (; ad < endPair; ad += 2 * stepBytes) {
@@ -316,6 +342,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kSegBy
             // doubles, the next prime loaded while this one strikes
             const double j0d = (double)j0;
             uint32_t pi = a.ncoop + a.ncarry * kThreads + tid, pn = 0;
+            // opaque per segment, so the compiler forms the prime and
+            // reciprocal addresses here instead of holding them (spilled, in
+            // the 64-register kernel) through the whole run
+            asm volatile("" : "+v"(pi));
             double ivn = 0.0;
             if (pi < a.nprimes) {
                 pn = a.primes[pi];
@@ -397,7 +427,7 @@ struct ffx_sieve_plan {
     std::vector<uint32_t> hostPrimes;  // the struck primes (> 61)
     uint32_t segBytes, blocksPerCU, cus;
     uint32_t laneCarry;   // at most this many carried lane primes per thread (FFX_SIEVE_LANE_CARRY)
-    uint32_t holdPrimes;  // 64 KiB segments: 0 by the lane-prime count, 1 held, 2 re-read
+    uint32_t holdPrimes;  // large segments: 0 by the lane-prime count, 1 held, 2 re-read
     uint32_t *dPrimes;
     double *dInv;
     uint32_t *dPatterns;
@@ -478,7 +508,8 @@ static int launch(const ffx_sieve_plan *pl, uint8_t *dst, uint64_t dstBase, uint
     // primes with more than 64 strikes per segment strike wave-cooperatively
     a.ncoop = (uint32_t)(std::lower_bound(pl->hostPrimes.begin(), pl->hostPrimes.end(), (uint32_t)(segOdds / 64)) -
                          pl->hostPrimes.begin());
-    if (a.ncoop > segOdds / 256) return fail(FFX_EINVAL, "sieve: %u cooperative primes exceed the table", a.ncoop);
+    const uint32_t coopCap = segBytes == kSmallMapSegBytes ? coop_capacity(kSmallMapSegBytes * 8) : coop_capacity(kLargeMapSegBytes * 8);
+    if (a.ncoop > coopCap) return fail(FFX_EINVAL, "sieve: %u cooperative primes exceed the table", a.ncoop);
     a.patterns = pl->dPatterns;
     a.segFirst = byteLo / segBytes;
     a.segCount = (byteHi + segBytes - 1) / segBytes - a.segFirst;
@@ -496,11 +527,11 @@ static int launch(const ffx_sieve_plan *pl, uint8_t *dst, uint64_t dstBase, uint
                            as_stream(stream), a);
     } else if (hold) {
         a.ncarry = std::min<uint32_t>(pl->laneCarry, kCarryHeld);
-        hipLaunchKernelGGL((sieve_segments<65536, kCarryHeld, true>), dim3((uint32_t)grid), dim3(kThreads), 0,
+        hipLaunchKernelGGL((sieve_segments<kLargeMapSegBytes, kCarryHeld, true>), dim3((uint32_t)grid), dim3(kThreads), 0,
                            as_stream(stream), a);
     } else {
         a.ncarry = std::min<uint32_t>(pl->laneCarry, kCarryReread);
-        hipLaunchKernelGGL((sieve_segments<65536, kCarryReread, false>), dim3((uint32_t)grid), dim3(kThreads), 0,
+        hipLaunchKernelGGL((sieve_segments<kLargeMapSegBytes, kCarryReread, false>), dim3((uint32_t)grid), dim3(kThreads), 0,
                            as_stream(stream), a);
     }
     FFX_HIP(hipGetLastError());
@@ -558,7 +589,8 @@ int ffx_sieve_plan_set(ffx_sieve_plan *plan, int param, uint32_t value)
     if (!plan) return fail(FFX_EINVAL, "sieve plan: null");
     switch (param) {
     case FFX_SIEVE_SEGMENT_BYTES:
-        if (value != 32768 && value != 65536) return fail(FFX_ERANGE, "segment bytes %u: 32768 or 65536", value);
+        if (value != kSmallMapSegBytes && value != kLargeMapSegBytes)
+            return fail(FFX_ERANGE, "segment bytes %u: %u or %u", value, kSmallMapSegBytes, kLargeMapSegBytes);
         plan->segBytes = value;
         return FFX_OK;
     case FFX_SIEVE_BLOCKS_PER_CU:

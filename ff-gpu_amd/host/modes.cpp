@@ -990,7 +990,7 @@ int runAutotune(const Config &cfg, const AutotuneOptions &opt)
     if ((rc = ffx_sieve_map_async(sp, dChk, checkBytes, stream)) || (rc = ffx_stream_sync(stream)) ||
         (rc = ffx_copy_d2h(chkRef.data(), dChk, checkBytes)))
         return fail("sieve reference", rc);
-    for (uint32_t seg : {32768u, 65536u})
+    for (uint32_t seg : {32768u, 114688u})
         for (uint32_t bpc : {1u, 2u, 3u, 4u}) {
             if ((rc = ffx_sieve_plan_set(sp, FFX_SIEVE_SEGMENT_BYTES, seg)) ||
                 (rc = ffx_sieve_plan_set(sp, FFX_SIEVE_BLOCKS_PER_CU, bpc)))
@@ -1012,7 +1012,7 @@ int runAutotune(const Config &cfg, const AutotuneOptions &opt)
         }
     const SieveTrial bs = *std::min_element(st.begin(), st.end(), [](auto &a, auto &b) { return a.ms < b.ms; });
     // the search reads the map the default configuration wrote
-    if ((rc = ffx_sieve_plan_set(sp, FFX_SIEVE_SEGMENT_BYTES, 65536)) ||
+    if ((rc = ffx_sieve_plan_set(sp, FFX_SIEVE_SEGMENT_BYTES, 114688)) ||
         (rc = ffx_sieve_map_async(sp, dMap, mapBytes, stream)) || (rc = ffx_stream_sync(stream)))
         return fail("sieve", rc);
 

@@ -138,12 +138,12 @@ int ffx_sieve_plan_destroy(ffx_sieve_plan *plan);
 
 /* Tunable sieve parameters (no reference counterpart; the reference fixes
  * its kernel geometry at compile time, source/sieve_slab_kernel.h:40-67). */
-#define FFX_SIEVE_SEGMENT_BYTES 1  /* LDS tile: map bytes per segment, 32768 or 65536 */
+#define FFX_SIEVE_SEGMENT_BYTES 1  /* LDS tile: map bytes per segment, 32768 or 114688 */
 #define FFX_SIEVE_BLOCKS_PER_CU 2  /* persistent grid: 1024-thread workgroups per CU (1..8) */
 #define FFX_SIEVE_LANE_CARRY 3     /* at most this many one-per-lane primes per thread keep their next
                                       strike across segments (0..64, default 64; the kernel holds
-                                      at most 12 with 32 KiB segments, 32 or 64 with 64 KiB) */
-#define FFX_SIEVE_HOLD_PRIMES 4    /* 64 KiB segments: carried primes held in registers (1) or
+                                      at most 12 with 32 KiB segments, 32 or 64 with 112 KiB) */
+#define FFX_SIEVE_HOLD_PRIMES 4    /* 112 KiB segments: carried primes held in registers (1) or
                                       re-read each segment (2); 0 (default) picks by their count */
 int ffx_sieve_plan_set(ffx_sieve_plan *plan, int param, uint32_t value);
 int ffx_sieve_plan_get(const ffx_sieve_plan *plan, int param, uint32_t *value);

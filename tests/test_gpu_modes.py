@@ -177,7 +177,7 @@ def test_autotune_writes_a_tune_file_that_keeps_stdout():
         p = ff_sieve("--autotune", "--opt", "sum_limit=65535", "--opt", "tune_reps=1", "--opt", f"out={f}")
         assert p.returncode == 0, p.stderr.decode()[-2000:]
         t = json.load(open(f))
-        assert t["sieve_segment_bytes"] in (32768, 65536) and 1 <= t["search_blocks_per_cu"] <= 16
+        assert t["sieve_segment_bytes"] in (32768, 114688) and 1 <= t["search_blocks_per_cu"] <= 16
         assert len(t["trials"]["search"]) == 36 and len(t["trials"]["sieve"]) == 8
         p = ff_sieve("--tune", f, 5, 65535)
         assert p.returncode == 0

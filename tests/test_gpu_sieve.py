@@ -83,8 +83,8 @@ def test_map_grown_in_ranges_equals_full_map():
 # carried lane primes covering none, part or all of the 3,500 sieving primes
 # of a 1 GiB map (1024 threads per workgroup), so both the carried strikes
 # and the per-segment first multiples run
-CARRY_CASES = [(32768, 0, 0), (32768, 0, 1), (32768, 0, 64), (65536, 1, 0), (65536, 1, 2), (65536, 1, 64),
-               (65536, 2, 0), (65536, 2, 3), (65536, 2, 64)]
+CARRY_CASES = [(32768, 0, 0), (32768, 0, 1), (32768, 0, 64), (114688, 1, 0), (114688, 1, 2), (114688, 1, 64),
+               (114688, 2, 0), (114688, 2, 3), (114688, 2, 64)]
 
 
 @pytest.mark.parametrize("seg,hold,carry", CARRY_CASES)
