@@ -736,7 +736,10 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
                 bool ok = !(kj & 1) || (((kNot3 >> dr) & (kNot3 >> orr)) & 1);
                 if (!tinySum) ok = ok && ((kCoprime15 >> xr) & 1) && (((kCoprime15 >> yr) & 1) || (int)t == dstarT);
                 if (ok) {
-                    segOffs[lane * 16 + c] = (uint8_t)(2 * t + 1);
+                    // bit 7: a Power2Even split of this class has 3 !| d*o,
+                    // so o must be prime (:619); d = 2t+1 and o = orr mod 3
+                    const bool needO = !(kj & 1) && dr % 3 != 0 && orr % 3 != 0;
+                    segOffs[lane * 16 + c] = (uint8_t)((2 * t + 1) | (needO ? 0x80u : 0u));
                     ++c;
                     skip += 2 * t + 1 < dmin;
                     le += 2 * t + 1 <= rem;
@@ -790,7 +793,7 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
                 auto d_at = [&](uint32_t g) {
                     const uint32_t ii = g - e.x + (e.y >> 16);
                     const uint32_t q = c == 1 ? ii : __umulhi(ii, e.w);
-                    return 30 * q + segOffs[j0 * 16 + (ii - q * c)];
+                    return 30 * q + (segOffs[j0 * 16 + (ii - q * c)] & 0x7Fu);
                 };
                 const uint32_t d0 = d_at(base), d1 = d_at(gl);
                 if (j0 != sj || d0 < sLo || d1 >= sLo + 2 * kSieveBits) {
@@ -804,23 +807,18 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
 #ifdef FFX_PROBE_COUNTERS
         if (lane == 0) FFX_COUNT(1, total - base < 64u * kRoundSplits ? total - base : 64u * kRoundSplits);
 #endif
-        int segOf[kRoundSplits];
-#pragma unroll
-        for (int u = 0; u < kRoundSplits; ++u) {
-            const uint32_t g = base + 64u * u + lane;
+        // Split g of segment j, through the filters that need no map read;
+        // survivors go to the probe queue.
+        auto filter = [&](uint32_t g, int j, uint4 e) {
             const bool live = g < total;
-            int j = hint;
-            if (live)
-                while (j + 1 < nseg && g >= segs[j + 1].x) ++j;
-            segOf[u] = j;
-            const uint4 e = segs[j];
             const uint32_t k = e.y & 31, c = (e.y >> 8) & 15, ii = live ? g - e.x + (e.y >> 16) : 0u;
             const uint32_t q = c == 1 ? ii : __umulhi(ii, e.w);
-            const uint32_t d = 30 * q + segOffs[j * 16 + (ii - q * c)];
+            const uint32_t cls = segOffs[j * 16 + (ii - q * c)];
+            const uint32_t d = 30 * q + (cls & 0x7Fu);
             const uint32_t o = live ? s - (d << k) : 1;
             // Power2Even without 3 | d*o: o must be prime (Power2Odd lists
-            // hold only d with 3 !| d*o)
-            const bool needO = !(k & 1) && d % 3 != 0 && o % 3 != 0;
+            // hold only d with 3 !| d*o); fixed by d's class
+            const bool needO = cls >> 7;
             bool rx, ry, ro;
             probe_residues(tab, d, o, k, e.z, rx, ry, ro);
             if (tinySum) {  // uniform: only sums below 4096 have probe values <= the largest residue prime
@@ -840,8 +838,30 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
             if (lane == 0) FFX_COUNT(2, __popcll(km));
 #endif
             pn += __popcll(km);
+        };
+        // uniform: the segment of the round's first split, and whether the
+        // whole round lies in it (then no lane searches the segment table)
+        int jr = hint;
+        while (jr + 1 < nseg && base >= segs[jr + 1].x) ++jr;
+        const uint32_t glast = (base + 64 * kRoundSplits < total ? base + 64 * kRoundSplits : total) - 1;
+        if (jr + 1 >= nseg || glast < segs[jr + 1].x) {
+            const uint4 e = segs[jr];
+#pragma unroll
+            for (int u = 0; u < kRoundSplits; ++u) filter(base + 64u * u + lane, jr, e);
+            hint = jr;
+        } else {
+            int segOf[kRoundSplits];
+#pragma unroll
+            for (int u = 0; u < kRoundSplits; ++u) {
+                const uint32_t g = base + 64u * u + lane;
+                int j = hint;
+                if (g < total)
+                    while (j + 1 < nseg && g >= segs[j + 1].x) ++j;
+                segOf[u] = j;
+                filter(g, j, segs[j]);
+            }
+            hint = __shfl(segOf[kRoundSplits - 1], 63);
         }
-        hint = __shfl(segOf[kRoundSplits - 1], 63);
         const bool flush = base + 64 * kRoundSplits <= a.eager || base + 64 * kRoundSplits >= total;
         if (pn < 128 && !(flush && pn > 0)) continue;
         wave_sync_lds();
