@@ -622,41 +622,90 @@ __device__ __forceinline__ void probe_batch(const SearchArgs &a, const uint2 *pq
     }
 }
 
-// Marks, in bits (kSieveBits, bit b = odd d = dLo + 2b), the d of k-segment
-// splits of sum s whose 2^k + d*o - 2 or d + o*2^k - 2 has a sieve prime
-// factor (o = s - d*2^k).  Lane 3i+r takes prime i: r = 0 the root of the
-// linear value, r = 1, 2 the roots (sm +- sqrt(D)) / 2^(k+1) of the
-// quadratic 2^k d^2 - s d - (2^k - 2) = 0 with D = s^2 + 4 * 2^k (2^k - 2).
-using lds_u32 = __attribute__((address_space(3))) uint32_t;
-__device__ __noinline__ void sieve_chunk(const uint32_t *sieveK, const uint8_t *sieveSqrt, uint32_t s, uint32_t k,
-                                        uint32_t dLo, lds_u32 *bits, int lane)
+// The sieve over d of one sum.  Lane 3i+r (i < kNSievePrimes) owns the item
+// (prime q_i, value r): r = 0 the root of the linear value d + o*2^k - 2,
+// r = 1, 2 the roots (sm +- sqrt(D)) / 2^(k+1) of the quadratic
+// 2^k d^2 - s d - (2^k - 2) = 0 (2^k + d*o - 2 = 0 mod q) with
+// D = s^2 + 4 * 2^k (2^k - 2).  s mod q is taken once per sum, the root once
+// per k-segment; each chunk then only places it relative to the chunk's
+// first d.
+struct SieveItem {
+    uint32_t q = 1, half = 0;  // the item's prime; kSieveBits/2 mod q
+    uint32_t sm = 0;           // s mod q
+    uint32_t root = 0;         // d = root (mod q) makes the item's value divisible by q
+    uint32_t kind = 0;         // 0: no root, 1: root, 2: every d (4^k = 1 and s 2^k = 2 mod q)
+};
+
+__device__ __forceinline__ void sieve_item_for_sum(SieveItem &it, uint32_t s, int lane)
 {
+    if (lane < 3 * kNSievePrimes) {
+        it.q = kSievePrimes[lane / 3];
+        it.half = (kSieveBits / 2) % it.q;
+        it.sm = s % it.q;
+    }
+}
+
+// (root, kind) of the lane's item for 2^k
+__device__ __noinline__ uint2 sieve_item_for_k(uint32_t q, uint32_t sm, const uint32_t *sieveK,
+                                               const uint8_t *sieveSqrt, uint32_t k, int lane)
+{
+    uint2 it = make_uint2(0u, 0u);
+    if (lane >= 3 * kNSievePrimes) return it;
+    const int pi = lane / 3, which = lane - 3 * pi;
+    const uint32_t rec = sieveK[pi * 32 + k];
+    const uint32_t pk = rec & 255, iY = (rec >> 8) & 255, iX = (rec >> 16) & 255;
+    if (which == 0) {
+        const uint32_t rhs = (2 + q - (sm * pk) % q) % q;  // d (1 - 4^k) = 2 - s 2^k
+        if (rec >> 24) it = make_uint2((rhs * iY) % q, 1u);
+        else if (rhs == 0) it.y = 2;  // 4^k = 1 mod q: the value is s 2^k - 2 for every d
+    } else {
+        const uint32_t D = (sm * sm + 4 * pk * ((pk + q - 2) % q)) % q;
+        const uint32_t r = sieveSqrt[kSieveSqrtOff.v[pi] + D];
+        if (r != 0xFFu) it = make_uint2((((which == 1 ? sm + r : sm + q - r) % q) * iX) % q, 1u);
+    }
+    return it;
+}
+
+// Marks, in bits (kSieveBits, bit b = odd d = dLo + 2b), the d of the
+// k-segment's splits whose 2^k + d*o - 2 or d + o*2^k - 2 has a sieve prime
+// factor (o = s - d*2^k).  Items of small primes mark many more bits than
+// those of large ones, so lane l marks the first half of its own item's
+// bits and the second half of item 62-l's: every lane then carries a small
+// and a large prime.
+using lds_u32 = __attribute__((address_space(3))) uint32_t;
+__device__ __noinline__ void sieve_chunk(const SieveItem it, uint32_t dLo, lds_u32 *bits, int lane)
+{
+    static_assert(3 * kNSievePrimes <= 63, "items fit lanes 0..62, paired l <-> 62-l");
+    constexpr uint32_t kHalf = kSieveBits / 2;
     for (uint32_t i = (uint32_t)lane; i < kSieveBits / 32; i += 64) bits[i] = 0;
+    // own item: first bit b0 < q with dLo + 2 b0 = root (mod q), odd d
+    uint32_t b0 = 0;
+    if (it.kind == 1) {
+        uint32_t t = it.root + it.q - dLo % it.q;  // d - dLo = t (mod q), t in [1, 2q)
+        if (t >= it.q) t -= it.q;
+        if (t & 1) t += it.q;  // odd d (dLo is odd): an even offset
+        b0 = t >> 1;
+    }
+    // the partner item's second half
+    const int partner = 62 - lane;
+    const uint32_t qB = __shfl(it.q, partner), hB = __shfl(it.half, partner), kindB = __shfl(it.kind, partner);
+    const uint32_t b0B = __shfl(b0, partner);
     wave_sync_lds();
-    for (int item = lane; item < 3 * kNSievePrimes; item += 64) {
-        const int pi = item / 3, which = item - 3 * pi;
-        const uint32_t q = kSievePrimes[pi];
-        const uint32_t rec = sieveK[pi * 32 + k];
-        const uint32_t pk = rec & 255, iY = (rec >> 8) & 255, iX = (rec >> 16) & 255;
-        const uint32_t sm = s % q;
-        int root = -1;
-        bool all = false;
-        if (which == 0) {
-            const uint32_t rhs = (2 + q - (sm * pk) % q) % q;  // d (1 - 4^k) = 2 - s 2^k
-            if (rec >> 24) root = (int)((rhs * iY) % q);
-            else all = rhs == 0;  // 4^k = 1 mod q: the value is s 2^k - 2 for every d
-        } else {
-            const uint32_t D = (sm * sm + 4 * pk * ((pk + q - 2) % q)) % q;
-            const uint32_t r = sieveSqrt[kSieveSqrtOff.v[pi] + D];
-            if (r != 0xFFu) root = (int)((((which == 1 ? sm + r : sm + q - r) % q) * iX) % q);
-        }
-        if (all) {
-            for (uint32_t i = 0; i < kSieveBits / 32; ++i)
+    if (lane < 63) {
+        if (it.kind == 2) {
+            for (uint32_t i = 0; i < kHalf / 32; ++i)
                 __hip_atomic_fetch_or(&bits[i], 0xFFFFFFFFu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-        } else if (root >= 0) {
-            uint32_t t = ((uint32_t)root + q - dLo % q) % q;  // first d >= dLo with d = root mod q
-            if (!((dLo + t) & 1)) t += q;                     // odd d (dLo is odd)
-            for (uint32_t b = t >> 1; b < kSieveBits; b += q)
+        } else if (it.kind == 1) {
+            for (uint32_t b = b0; b < kHalf; b += it.q)
+                __hip_atomic_fetch_or(&bits[b >> 5], 1u << (b & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        }
+        if (kindB == 2) {
+            for (uint32_t i = kHalf / 32; i < kSieveBits / 32; ++i)
+                __hip_atomic_fetch_or(&bits[i], 0xFFFFFFFFu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        } else if (kindB == 1) {
+            // first b >= kHalf with b = b0B (mod qB): kHalf + ((b0B - kHalf) mod qB)
+            uint32_t b = kHalf + (b0B >= hB ? b0B - hB : b0B + qB - hB);
+            for (; b < kSieveBits; b += qB)
                 __hip_atomic_fetch_or(&bits[b >> 5], 1u << (b & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
         }
     }
@@ -779,6 +828,8 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
     int sj = -1;
     uint32_t sLo = 0;
     const bool sieveSum = s >= kSieveMinSum;
+    SieveItem item;
+    if (sieveSum) sieve_item_for_sum(item, s, lane);
     for (uint32_t base = 0; base < total; base += 64 * kRoundSplits) {
         bool useSieve = false;
         if (sieveSum && base >= a.eager) {
@@ -797,7 +848,12 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
                 };
                 const uint32_t d0 = d_at(base), d1 = d_at(gl);
                 if (j0 != sj || d0 < sLo || d1 >= sLo + 2 * kSieveBits) {
-                    sieve_chunk(a.sieveK, a.sieveSqrt, s, e.y & 31, d0, (lds_u32 *)dbits, lane);
+                    if (j0 != sj) {
+                        const uint2 rk = sieve_item_for_k(item.q, item.sm, a.sieveK, a.sieveSqrt, e.y & 31, lane);
+                        item.root = rk.x;
+                        item.kind = rk.y;
+                    }
+                    sieve_chunk(item, d0, (lds_u32 *)dbits, lane);
                     sj = j0;
                     sLo = d0;
                 }
