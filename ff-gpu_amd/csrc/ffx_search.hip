@@ -84,14 +84,15 @@ constexpr int kMaxSegs = 32;
 constexpr uint32_t kEagerSplits = 1024;  // the autotuned choice for odd sums to 2^20 and 2^21
 constexpr uint32_t kBlocksPerCU = 4;  // default persistent grid: CUs x this (all resident)
 
-// Residue filter: a probe value larger than the largest prime of the moduli
-// in use that shares a factor with one of them is composite without touching
-// the map.  One bit table per modulus, bit r set when gcd(r, m) == 1.  Each
-// modulus is below 2^16, so
+// Residue filter: a probe value that shares a factor with one of the moduli
+// in use and is not that factor itself is composite without touching the
+// map.  One bit table per modulus, bit r set when gcd(r, m) == 1 or r is a
+// prime factor of m: a value with a clear bit is then composite at any size
+// (a set bit only means "not ruled out": a value congruent to a prime factor
+// of m goes on to the map read).  Each modulus is below 2^16, so
 // residues of the two probe values 2^k + d*o - 2 and d + o*2^k - 2 come from
 // the 32-bit residues of d, o and 2^k with 32-bit products.
 constexpr uint32_t kMods[5] = {3 * 5 * 7 * 11 * 13, 17 * 19 * 23, 29 * 31, 37 * 41, 43 * 47};
-constexpr uint32_t kModTopPrime[5] = {13, 23, 31, 41, 47};
 // Two moduli measured fastest on MI355X with the mod-30 split classes, the
 // sieve over d and the AllButOne pre-check: [5, 2^20) search 11.4 ms with
 // two, 11.9 with one or three; [5, 2^21) 45.2 ms with two, 46.3 with three,
@@ -105,7 +106,6 @@ constexpr int kNMods = FFX_RESIDUE_MODULI;
 static_assert(kNMods >= 1 && kNMods <= 5, "1..5 residue moduli");
 constexpr uint32_t tab_offset(int i) { return i == 0 ? 0 : tab_offset(i - 1) + (kMods[i - 1] + 31) / 32; }
 constexpr uint32_t kTabWords = tab_offset(kNMods);
-constexpr uint32_t kLargestResiduePrime = kModTopPrime[kNMods - 1];
 
 // Sieve over d: in a long k-segment, a chunk of kSieveBits consecutive odd d
 // is marked where 2^k + d*o - 2 (quadratic in d: two residue classes mod q
@@ -326,9 +326,8 @@ __device__ __forceinline__ void residue_all(const uint32_t *tab, uint32_t d, uin
     }
 }
 
-// rx (ry, ro) false when X (Y, o) has a prime factor in
-// 3..kLargestResiduePrime; values above that prime are then composite.
-// pms = pow2_residues(k).
+// rx (ry, ro) false when X (Y, o) is composite by a prime factor of the
+// moduli (a multiple of that prime other than the prime itself).  pms = pow2_residues(k).
 __device__ __forceinline__ void probe_residues(const uint32_t *tab, uint32_t d, uint32_t o, uint32_t k, uint32_t pms,
                                                bool &rx, bool &ry, bool &ro)
 {
@@ -514,15 +513,15 @@ __device__ __forceinline__ void drain(const SearchArgs &a, const uint32_t *tab, 
             // one term of p's pair, then its other term, then p''s pair, each
             // read only while the split is still in (fewer reads: the kernel
             // is bound by them)
-            // a term with a prime factor in 3..13 that is larger than 13 is
-            // composite: rejected before any read (C2: 104 M of these reads
-            // per launch down to 19 M, 11.96 -> 11.88 ms)
+            // a term with a clear residue bit is composite: rejected before
+            // any read (C2: 104 M of these reads per launch down to 19 M,
+            // 11.96 -> 11.88 ms)
             bool r[4] = {true, true, true, true};
             pair_residues(tab, p, c1, k, r[0], r[1]);
             if (two) pair_residues(tab, p2, c2, k, r[2], r[3]);
 #pragma unroll
             for (int i = 0; i < 4; ++i)
-                if (u[i] && t[i] != own && t[i] > kModTopPrime[0] && !r[i]) maybe = false;
+                if (u[i] && t[i] != own && !r[i]) maybe = false;
             uint32_t b[4];
             bool use[4];
 #pragma unroll
@@ -749,12 +748,11 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
     //  * Power2Odd (k odd) needs 3 !| d and 3 !| o (its family, :578);
     //  * both probe values X = 2^k + d*o - 2 and Y = d + o*2^k - 2 must be
     //    prime, so neither may be divisible by 3 or 5 -- for s >= 4096 both
-    //    exceed 47 (see tinySum), and Y is not tested for the one split with
+    //    exceed 5 (see tinySum), and Y is not tested for the one split with
     //    d = o, whose class is kept whole.
     // That drops ~60% of the splits before any per-split work.
     // For s >= 4096 every split has X, Y > 47: if o >= 16, d*o >= 48 and
     // o*2^k >= 64; otherwise d*2^k > 4080, so 2^k >= 64 or d >= 64.
-    static_assert(kLargestResiduePrime < 48, "tinySum bound");
     const bool tinySum = s < 4096;
     const uint32_t t3 = (s - 3) / 3;
     const int kTop = t3 >= 4 ? 31 - __clz(t3) : 0;  // largest k with 3*2^k <= s-3
@@ -877,12 +875,7 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
             const bool needO = cls >> 7;
             bool rx, ry, ro;
             probe_residues(tab, d, o, k, e.z, rx, ry, ro);
-            if (tinySum) {  // uniform: only sums below 4096 have probe values <= the largest residue prime
-                rx = rx || (1ull << k) + (uint64_t)d * o - 2 <= kLargestResiduePrime;
-                ry = ry || (uint64_t)d + ((uint64_t)o << k) - 2 <= kLargestResiduePrime;
-            }
             ry = ry || d == o;
-            ro = ro || o <= kLargestResiduePrime;
             bool keep = live && rx && ry && (!needO || ro);
             if (useSieve && keep && d != o) {
                 const uint32_t b = (d - sLo) >> 1;
@@ -1149,7 +1142,11 @@ static SearchTables build_search_tables(uint32_t sumLimitMax)
                 g = m;
                 m = tmp;
             }
-            if (g == 1) t.coprime[tab_offset(i) + r / 32] |= 1u << (r % 32);
+            // coprime, or a prime factor of the modulus (so a probe value
+            // that is one of those primes is not ruled out)
+            bool factor = r > 1 && kMods[i] % r == 0;
+            for (uint32_t f = 2; factor && f * f <= r; ++f) factor = r % f != 0;
+            if (g == 1 || factor) t.coprime[tab_offset(i) + r / 32] |= 1u << (r % 32);
         }
     t.sieveK.assign(kNSievePrimes * 32, 0);
     t.sieveSqrt.assign(kSieveSqrtBytes, 0xFF);
