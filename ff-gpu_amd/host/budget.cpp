@@ -1,6 +1,7 @@
 #include "budget.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <limits>
 
@@ -32,14 +33,38 @@ DeviceBudget shrinkFractionForAlloc(uint64_t freeBytes, const Config &cfg, doubl
     return computeBudget(freeBytes, cfg, 0.5 * (fraction + floorF));
 }
 
+// Sizes in the units of the reference's messages (source/budget.cpp:53-68):
+// the largest binary unit the count reaches, two decimals for GiB, one for
+// MiB and KiB, whole bytes below.
 std::string bytesToHuman(uint64_t bytes)
 {
-    char buf[64];
-    if (bytes >= (1ull << 30)) std::snprintf(buf, sizeof buf, "%.2f GiB", (double)bytes / (double)(1ull << 30));
-    else if (bytes >= (1ull << 20)) std::snprintf(buf, sizeof buf, "%.1f MiB", (double)bytes / (double)(1ull << 20));
-    else if (bytes >= (1ull << 10)) std::snprintf(buf, sizeof buf, "%.1f KiB", (double)bytes / 1024.0);
-    else std::snprintf(buf, sizeof buf, "%llu B", (unsigned long long)bytes);
-    return buf;
+    static const struct {
+        int shift, decimals;
+        const char *unit;
+    } kUnits[] = {{30, 2, "GiB"}, {20, 1, "MiB"}, {10, 1, "KiB"}};
+    char text[48];
+    for (const auto &u : kUnits)
+        if (bytes >> u.shift) {
+            std::snprintf(text, sizeof text, "%.*f %s", u.decimals, std::ldexp((double)bytes, -u.shift), u.unit);
+            return text;
+        }
+    return std::to_string(bytes) + " B";
+}
+
+TierPlan planTier(uint64_t backing, uint64_t hostTierBytes, uint64_t mapM, uint64_t slabBytes)
+{
+    TierPlan t;
+    t.mapBytes = (mapM >> 4) + 1;
+    t.capacity = hostTierBytes > std::numeric_limits<uint64_t>::max() - backing ? std::numeric_limits<uint64_t>::max()
+                                                                                : backing + hostTierBytes;
+    t.spill = t.mapBytes > backing;
+    // three slab buffers when the backing holds them, else two of at most
+    // half the backing; whole 4 KiB pages
+    t.ring = slabBytes && backing / slabBytes >= 3 ? 3 : 2;
+    t.ringSlabBytes = std::min(slabBytes ? slabBytes : backing, backing / (uint64_t)t.ring) & ~4095ull;
+    t.gatePass = t.capacity >= t.mapBytes && (!t.spill || t.ringSlabBytes > 0);
+    t.searchMapValue = !t.spill ? mapM : backing ? mapValueFor(backing * 16 - 1, 0) : 15;
+    return t;
 }
 
 uint64_t mapValueFor(uint64_t maxM, uint64_t cap)

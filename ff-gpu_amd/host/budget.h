@@ -27,6 +27,22 @@ DeviceBudget computeBudget(uint64_t freeBytes, const Config &cfg, double fractio
 DeviceBudget shrinkFractionForAlloc(uint64_t freeBytes, const Config &cfg, double fraction);
 std::string bytesToHuman(uint64_t bytes);
 
+// How a leg's map of mapM values (mapBytes = mapM/16 + 1) fits one device
+// with a host tier behind it.  The run-gate is the reference's aggregate one
+// (source/main.cpp:296-334): backing + host tier >= mapBytes.  A map larger
+// than the backing spills: --dump-map streams it through a ring of device
+// slab buffers into host memory (ringSlabBytes * ring <= backing), and the
+// search keeps the largest map the backing holds on the device, Miller-Rabin
+// answering above it (the same answers).
+struct TierPlan {
+    uint64_t mapBytes = 0, capacity = 0;
+    bool gatePass = false, spill = false;
+    uint64_t ringSlabBytes = 0;
+    int ring = 0;
+    uint64_t searchMapValue = 0;
+};
+TierPlan planTier(uint64_t backing, uint64_t hostTierBytes, uint64_t mapM, uint64_t slabBytes);
+
 // Device map value for a map that must answer values up to maxM: maxM, or
 // the --map-cap bound when smaller (Miller-Rabin answers above the map), as
 // a whole number of map bytes (16*bytes - 1), at least 15.

@@ -83,6 +83,17 @@ int main(int argc, char **argv)
                     b.fraction, b.freeUsed, b.budget, b.scratch, b.headroom, b.backing, b.slabCount);
         return 0;
     }
+    if (cmd == "tier" && argc >= 6) {  // backing hostTier mapM slab
+        const TierPlan t = planTier(u64(argv[2]), u64(argv[3]), u64(argv[4]), u64(argv[5]));
+        std::printf("mapBytes %" PRIu64 "\ncapacity %" PRIu64 "\ngatePass %d\nspill %d\nringSlabBytes %" PRIu64
+                    "\nring %d\nsearchMapValue %" PRIu64 "\n",
+                    t.mapBytes, t.capacity, (int)t.gatePass, (int)t.spill, t.ringSlabBytes, t.ring, t.searchMapValue);
+        return 0;
+    }
+    if (cmd == "human" && argc >= 3) {
+        std::printf("%s\n", bytesToHuman(u64(argv[2])).c_str());
+        return 0;
+    }
     if (cmd == "size" && argc >= 3) {
         uint64_t v = 0;
         if (!readByteSize(argv[2], &v)) return 1;

@@ -126,21 +126,34 @@ int runLeg(const Config &cfg, const std::vector<std::string> &pos)
         return 1;
     }
     const uint64_t mapM = deviceMapValue(sg, cfg.mapCap);
-    const uint64_t mapBytes = (mapM >> 4) + 1;
-    const uint64_t capacity = b.backing + cfg.hostTierCapBytes;
-    std::fprintf(stderr, "[ff_sieve] aggregate: backing=%llu B hostTier=%llu B mapBytes=%llu B (%s) -> %s\n",
+    const TierPlan tier = planTier(b.backing, cfg.hostTierCapBytes, mapM, cfg.slabSizeBytes);
+    const uint64_t mapBytes = tier.mapBytes;
+    std::fprintf(stderr,
+                 "[ff_sieve] aggregate: backing=%llu B hostTier=%llu B capacity=%llu B (%s) mapBytes=%llu B (%s) "
+                 "-> %s\n",
                  (unsigned long long)b.backing, (unsigned long long)cfg.hostTierCapBytes,
-                 (unsigned long long)mapBytes, bytesToHuman(mapBytes).c_str(),
-                 capacity >= mapBytes && b.backing >= mapBytes ? "GATE PASS" : "GATE FAIL");
-    if (capacity < mapBytes || b.backing < mapBytes) {
+                 (unsigned long long)tier.capacity, bytesToHuman(tier.capacity).c_str(), (unsigned long long)mapBytes,
+                 bytesToHuman(mapBytes).c_str(), tier.gatePass ? "GATE PASS" : "GATE FAIL");
+    if (!tier.gatePass) {
         std::fprintf(stderr,
-                     "[ff_sieve] ERROR: no device fits leg %u (map %llu B = %s): capacity %llu B\n"
-                     "[ff_sieve] remediation: raise FF_VRAM_BUDGET, or bound the device map with --map-cap "
-                     "(Miller-Rabin answers above it; stdout is unchanged)\n",
+                     "[ff_sieve] ERROR: no device fits leg %u (map %llu B = %s): capacity %llu B (%s)\n"
+                     "[ff_sieve] remediation: raise FF_VRAM_BUDGET and/or --host-tier-cap, or bound the device "
+                     "map with --map-cap (Miller-Rabin answers above it; stdout is unchanged)\n",
                      sumLimit, (unsigned long long)mapBytes, bytesToHuman(mapBytes).c_str(),
-                     (unsigned long long)b.backing);
+                     (unsigned long long)tier.capacity, bytesToHuman(tier.capacity).c_str());
         return 1;
     }
+    const bool dumping = !cfg.dumpMapFile.empty();
+    // the search's device map: the whole map, or with a spilled map the
+    // largest the backing holds (Miller-Rabin above it, same answers)
+    const uint64_t searchM = dumping ? mapM : tier.searchMapValue;
+    const uint64_t deviceBytes = dumping && tier.spill ? 0 : (searchM >> 4) + 1;
+    if (tier.spill)
+        std::fprintf(stderr,
+                     "[ff_sieve] host tier: map %s > backing %s; %s\n", bytesToHuman(mapBytes).c_str(),
+                     bytesToHuman(b.backing).c_str(),
+                     dumping ? "the map streams through device slab buffers into host memory"
+                             : "the search's device map holds values up to its bound, Miller-Rabin answers above");
 
     for (const std::string &h : sg.headers) std::printf("%s\n", h.c_str());
     std::fflush(stdout);
@@ -150,39 +163,59 @@ int runLeg(const Config &cfg, const std::vector<std::string> &pos)
     void *stream = ffx_stream_create();
     ffx_sieve_plan *sp = nullptr;
     void *dMap = nullptr;
+    uint8_t *pinnedMap = nullptr;
     AtExit release([&] {
         if (stream) ffx_stream_sync(stream);
         ffx_free(dMap);
+        ffx_free_pinned(pinnedMap);
         ffx_sieve_plan_destroy(sp);
         ffx_stream_destroy(stream);
     });
     if (!stream) return err("stream", FFX_EIO);
-    if ((rc = ffx_sieve_plan_create(&sp, mapM))) return err("sieve plan", rc);
+    if ((rc = ffx_sieve_plan_create(&sp, deviceBytes ? searchM : mapM))) return err("sieve plan", rc);
     if (cfg.sieveSegmentBytes && (rc = ffx_sieve_plan_set(sp, FFX_SIEVE_SEGMENT_BYTES, cfg.sieveSegmentBytes)))
         return err("sieve tuning", rc);
     if (cfg.sieveBlocksPerCU && (rc = ffx_sieve_plan_set(sp, FFX_SIEVE_BLOCKS_PER_CU, cfg.sieveBlocksPerCU)))
         return err("sieve tuning", rc);
-    dMap = ffx_alloc(((mapBytes + 255) & ~255ull));
-    if (!dMap) return err("map allocation", FFX_ENOMEM);
-    std::vector<uint8_t> hostMap;
-    if (!cfg.dumpMapFile.empty()) hostMap.resize(mapBytes);
+    std::vector<uint8_t> pageableMap;
+    uint8_t *hostMap = nullptr;
+    if (dumping) {
+        // pinned, so the slab copies run at the link's rate; pageable if the
+        // host cannot pin that much
+        pinnedMap = (uint8_t *)ffx_alloc_pinned(mapBytes);
+        if (!pinnedMap) {
+            std::fprintf(stderr, "[ff_sieve] dump-map: %s of pinned host memory refused; using pageable memory\n",
+                         bytesToHuman(mapBytes).c_str());
+            pageableMap.resize(mapBytes);
+        }
+        hostMap = pinnedMap ? pinnedMap : pageableMap.data();
+    }
     SieveTiming st;
-    rc = sieveMap(sp, dMap, mapBytes, cfg.dumpMapFile.empty() ? 0 : cfg.slabSizeBytes,
-                  cfg.dumpMapFile.empty() ? nullptr : hostMap.data(), stream, &st);
+    if (deviceBytes) {
+        dMap = ffx_alloc(((deviceBytes + 255) & ~255ull));
+        if (!dMap) return err("map allocation", FFX_ENOMEM);
+        rc = sieveMap(sp, dMap, deviceBytes, dumping ? cfg.slabSizeBytes : 0, hostMap, stream, &st);
+    } else {
+        rc = sieveMapToHost(sp, mapBytes, tier.ringSlabBytes, tier.ring, hostMap, stream, &st);
+    }
     if (rc) return err("sieve", rc);
     const double sieveWall = msSince(tSieve);
-    std::fprintf(stderr, "[ff_sieve] sieve: %llu slab(s), kernel %.3f ms, %.1f GB/s\n", (unsigned long long)st.slabs,
-                 st.kernelMs, (double)mapBytes / (st.kernelMs * 1e6));
+    std::fprintf(stderr, "[ff_sieve] sieve: %llu slab(s)%s, %.3f ms on the device, %.1f GB/s\n",
+                 (unsigned long long)st.slabs, deviceBytes ? "" : " through the host tier", st.kernelMs,
+                 (double)(deviceBytes ? deviceBytes : mapBytes) / (st.kernelMs * 1e6));
     phase("sieve phase", sieveWall);
 
-    if (!cfg.dumpMapFile.empty()) {
+    if (dumping) {
         FILE *fo = std::fopen(cfg.dumpMapFile.c_str(), "wb");
-        if (!fo || std::fwrite(hostMap.data(), 1, mapBytes, fo) != mapBytes) {
+        if (!fo || std::fwrite(hostMap, 1, mapBytes, fo) != mapBytes) {
             std::fprintf(stderr, "[ff_sieve] ERROR: cannot write %s\n", cfg.dumpMapFile.c_str());
             if (fo) std::fclose(fo);
             return 1;
         }
-        std::fclose(fo);
+        if (std::fclose(fo)) {
+            std::fprintf(stderr, "[ff_sieve] ERROR: cannot write %s\n", cfg.dumpMapFile.c_str());
+            return 1;
+        }
         std::fprintf(stderr, "[ff_sieve] dump-map: wrote %s (%llu bytes)\n", cfg.dumpMapFile.c_str(),
                      (unsigned long long)mapBytes);
         return 0;
@@ -220,7 +253,7 @@ int runLeg(const Config &cfg, const std::vector<std::string> &pos)
         e1 = ffx_event_create();
         if (!dOut || !dCount || !e0 || !e1) return err("search allocation", FFX_ENOMEM);
         ffx_event_record(e0, stream);
-        if ((rc = ffx_search_async(plan, dMap, mapM, lo, hi, dOut, dCount, stream))) return err("search", rc);
+        if ((rc = ffx_search_async(plan, dMap, searchM, lo, hi, dOut, dCount, stream))) return err("search", rc);
         ffx_event_record(e1, stream);
         uint32_t count = 0;
         if ((rc = ffx_copy_d2h_async(&count, dCount, 4, stream)) || (rc = ffx_stream_sync(stream)))

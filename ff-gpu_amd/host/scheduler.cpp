@@ -92,4 +92,52 @@ int sieveMap(const ffx_sieve_plan *plan, void *d_map, uint64_t mapBytes, uint64_
     return FFX_OK;
 }
 
+int sieveMapToHost(const ffx_sieve_plan *plan, uint64_t mapBytes, uint64_t slabBytes, int ring, uint8_t *hostOut,
+                   void *stream, SieveTiming *timing)
+{
+    if (!plan || !hostOut || !stream || ring < 1 || ring > 8 || slabBytes == 0) return FFX_EINVAL;
+    const auto t0 = std::chrono::steady_clock::now();
+    Events evs;
+    void *start = evs.make(), *stop = evs.make();
+    void *copy = ffx_stream_create();
+    std::vector<void *> buf((size_t)ring, nullptr), computed((size_t)ring, nullptr), copied((size_t)ring, nullptr);
+    int rc = start && stop && copy ? FFX_OK : FFX_EIO;
+    for (int r = 0; !rc && r < ring; ++r) {
+        buf[(size_t)r] = ffx_alloc(slabBytes);
+        computed[(size_t)r] = evs.make();
+        copied[(size_t)r] = evs.make();
+        if (!buf[(size_t)r]) rc = FFX_ENOMEM;
+        else if (!computed[(size_t)r] || !copied[(size_t)r]) rc = FFX_EIO;
+    }
+    uint64_t nslabs = 0;
+    if (!rc) rc = ffx_event_record(start, stream);
+    for (uint64_t lo = 0; !rc && lo < mapBytes; lo += slabBytes, ++nslabs) {
+        const uint64_t len = mapBytes - lo < slabBytes ? mapBytes - lo : slabBytes;
+        const size_t r = (size_t)(nslabs % (uint64_t)ring);
+        // the buffer is free once the copy of the slab it held has landed
+        if (nslabs >= (uint64_t)ring && (rc = ffx_event_wait(copied[r], stream))) break;
+        if ((rc = ffx_sieve_slab_async(plan, lo << 4, (lo + len) << 4, buf[r], stream))) break;
+        if ((rc = ffx_event_record(computed[r], stream))) break;
+        if ((rc = ffx_event_wait(computed[r], copy))) break;
+        if ((rc = ffx_copy_d2h_async(hostOut + lo, buf[r], len, copy))) break;
+        if ((rc = ffx_event_record(copied[r], copy))) break;
+    }
+    if (!rc) rc = ffx_event_record(stop, copy);
+    if (!rc) rc = ffx_event_sync(stop);
+    // drain both streams before the buffers go, also after an error
+    if (copy) ffx_stream_sync(copy);
+    ffx_stream_sync(stream);
+    for (void *b : buf) ffx_free(b);
+    ffx_stream_destroy(copy);
+    if (rc) return rc;
+    double ms = 0;
+    if ((rc = ffx_event_elapsed_ms(start, stop, &ms))) return rc;
+    if (timing) {
+        timing->kernelMs = ms;
+        timing->slabs = nslabs;
+        timing->wallMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return FFX_OK;
+}
+
 }  // namespace ffg

@@ -28,6 +28,18 @@ struct SieveTiming {
 int sieveMap(const ffx_sieve_plan *plan, void *d_map, uint64_t mapBytes, uint64_t slabBytes, uint8_t *hostOut,
              void *stream, SieveTiming *timing);
 
+// Host tier: sieves map bytes [0, mapBytes) into hostOut with no
+// device-resident map, for maps larger than the device backing
+// (source/pull_scheduler.cpp:164-200 computes such "homeless" slabs in a
+// staging buffer).  Slab i is computed into device buffer i % ring on
+// `stream` and copied into hostOut on a copy stream; the compute of slab
+// i + ring waits on the device for slab i's copy, so the host thread only
+// enqueues and compute overlaps the copies.  Device memory used:
+// ring * slabBytes.  hostOut should come from ffx_alloc_pinned for the copies
+// to run at the link's rate.
+int sieveMapToHost(const ffx_sieve_plan *plan, uint64_t mapBytes, uint64_t slabBytes, int ring, uint8_t *hostOut,
+                   void *stream, SieveTiming *timing);
+
 }  // namespace ffg
 
 #endif

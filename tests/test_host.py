@@ -115,6 +115,39 @@ def test_budget_kats():
     assert float(fl["fraction"]) == 0.50
 
 
+def test_tier_plan():
+    gib = 1 << 30
+    m21 = 17179869185 * 16 - 1  # the 2^21 leg's map value: 16 GiB + 1 byte of map
+    # the map fits the backing: no spill, the search keeps the whole map
+    t = kv("tier", 20 * gib, 0, m21, gib)
+    assert (t["gatePass"], t["spill"], int(t["searchMapValue"])) == ("1", "0", m21)
+    # a 2 GiB backing fails alone (source/main.cpp:296-334) and passes with a host tier
+    t = kv("tier", 2 * gib, 0, m21, 512 << 20)
+    assert (t["gatePass"], t["spill"]) == ("0", "1")
+    t = kv("tier", 2 * gib, 15 * gib, m21, 512 << 20)
+    assert int(t["capacity"]) == 17 * gib and t["gatePass"] == "1" and t["spill"] == "1"
+    # three 512 MiB slab buffers fit; the search's device map fills the backing
+    assert (t["ring"], int(t["ringSlabBytes"])) == ("3", 512 << 20)
+    assert int(t["searchMapValue"]) == 2 * gib * 16 - 1
+    # one byte short of the map is still a failure
+    t = kv("tier", 2 * gib, 17179869185 - 2 * gib - 1, m21, 512 << 20)
+    assert t["gatePass"] == "0"
+    # a slab larger than half the backing: two buffers of half, whole pages
+    t = kv("tier", 15 << 20, 1 << 40, 2 ** 30 - 1, 1 << 30)
+    assert (t["ring"], int(t["ringSlabBytes"])) == ("2", (15 << 19) // 4096 * 4096)
+    # the capacity saturates instead of wrapping
+    t = kv("tier", 2 * gib, 2 ** 64 - 1, m21, gib)
+    assert int(t["capacity"]) == 2 ** 64 - 1 and t["gatePass"] == "1"
+
+
+def test_bytes_to_human():
+    # the units and decimals of the reference's messages (source/budget.cpp:53-68)
+    for n, want in [(0, "0 B"), (1023, "1023 B"), (1024, "1.0 KiB"), (1536, "1.5 KiB"), ((1 << 20) - 1, "1024.0 KiB"),
+                    (1 << 20, "1.0 MiB"), (17179869185, "16.00 GiB"), (3 << 29, "1.50 GiB"), (2 ** 64 - 1,
+                                                                                           "17179869184.00 GiB")]:
+        assert hc("human", n).stdout.strip() == want, n
+
+
 def test_size_and_fraction_parsers():
     for s, v in [("20GiB", 21474836480), ("8GiB", 8589934592), ("1GiB", 1 << 30), ("1024MiB", 1 << 30),
                  ("64MiB", 67108864), ("4096", 4096), ("1.5GiB", 1610612736), ("0", 0)]:
