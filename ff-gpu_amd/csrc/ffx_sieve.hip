@@ -517,9 +517,12 @@ static int launch(const ffx_sieve_plan *pl, uint8_t *dst, uint64_t dstBase, uint
     const uint64_t maxGrid = (uint64_t)pl->cus * pl->blocksPerCU;
     uint64_t grid = a.segCount < maxGrid ? a.segCount : maxGrid;
     // each workgroup's run of segments stays below 2^31 odd values (its
-    // carried strike offsets are 32-bit)
+    // carried strike offsets are 32-bit); when that takes more workgroups
+    // than the grid, a whole multiple of the CU count keeps the last wave
+    // from running a few workgroups alone (70 GB map, one 112 KiB workgroup
+    // per CU: 261 runs took 357 ms, 512 take 188)
     const uint64_t minGrid = (a.segCount * segOdds + (1ull << 31) - 1) >> 31;
-    if (grid < minGrid) grid = minGrid;
+    if (grid < minGrid) grid = std::min<uint64_t>((minGrid + pl->cus - 1) / pl->cus * pl->cus, a.segCount);
     const bool hold = pl->holdPrimes == 1 || (pl->holdPrimes == 0 && a.nprimes - a.ncoop <= kHeldUpTo);
     if (segBytes == 32768) {
         a.ncarry = std::min<uint32_t>(pl->laneCarry, kCarrySmall);
