@@ -74,7 +74,8 @@ def random_read_ceiling(nbytes):
 
 COUNTER_NAMES = ["sums_enumerated", "splits_visited", "splits_past_residue_filter", "reads_2k_plus_do",
                  "reads_d_plus_o2k", "reads_o", "allbutone_checks", "reads_in_allbutone", "miller_rabin_tests",
-                 "splits_listed"]
+                 "splits_listed", "splits_visited_solutions", "splits_visited_early_exit", "splits_visited_no_split",
+                 "filter_rounds", "sieved_rounds"]
 
 
 def count_probes(workload):

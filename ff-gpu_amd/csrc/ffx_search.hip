@@ -44,6 +44,7 @@
 #include <cstring>
 #include <mutex>
 #include <new>
+#include <type_traits>
 #include <vector>
 
 #include "ffx_internal.h"
@@ -110,28 +111,36 @@ constexpr uint32_t kTabWords = tab_offset(kNMods);
 // Sieve over d: in a long k-segment, a chunk of kSieveBits consecutive odd d
 // is marked where 2^k + d*o - 2 (quadratic in d: two residue classes mod q
 // at most) or d + o*2^k - 2 (linear: one class) is divisible by a prime q
-// of kSievePrimes, one lane per (prime, value, root); splits on marked d are
-// dropped before any map read.  Both values exceed 2*sqrt(s-1) - 2 > 251
-// for every split of a sum s >= kSieveMinSum, so a marked value is
-// composite.  The chunk's bits live in 512 bytes of LDS per wave.
-constexpr uint32_t kSievePrimes[43] = {37,  41,  43,  47,  53,  59,  61,  67,  71,  73,  79,  83,  89,  97,  101,
-                                       103, 107, 109, 113, 127, 131, 137, 139, 149, 151, 157, 163, 167, 173, 179,
-                                       181, 191, 193, 197, 199, 211, 223, 227, 229, 233, 239, 241, 251};
-// Primes in use: 21 (37..131) measured as fast as 32 or 43 and faster than 14
-// or 8 (odd sums [5, 2^20): 16.5 / 16.6 / 16.5 / 16.9 / 17.1 ms).
-constexpr int kNSievePrimes = 21;  // (prime, value, root) items: 3 per prime
+// of kSievePrimes; splits on marked d are dropped before any map read, with
+// no per-split residue arithmetic (the split classes mod 30 already exclude
+// 3 and 5).  Both values exceed 2*sqrt(s-1) - 2 > 251 for every split of a
+// sum s >= kSieveMinSum, so a marked value is composite.  The primes 7..31
+// mark many bits each and are laid in as 64-bit word masks, every lane
+// building its own 64 bits from all their roots; the primes from 37 strike
+// their bits one lane per (prime, value, root).  The chunk's bits live in
+// 512 bytes of LDS per wave.
+constexpr uint32_t kSievePrimes[51] = {7,   11,  13,  17,  19,  23,  29,  31,  37,  41,  43,  47,  53,
+                                       59,  61,  67,  71,  73,  79,  83,  89,  97,  101, 103, 107, 109,
+                                       113, 127, 131, 137, 139, 149, 151, 157, 163, 167, 173, 179, 181,
+                                       191, 193, 197, 199, 211, 223, 227, 229, 233, 239, 241, 251};
+constexpr int kNSmallSieve = 8;   // 7..31: word masks
+static_assert(kSievePrimes[kNSmallSieve - 1] < 64 && kSievePrimes[kNSmallSieve] > 31, "masked primes below 64, struck from 37");
+// Struck primes in use: 21 (37..131) measured as fast as 32 or 43 and faster
+// than 14 or 8 (odd sums [5, 2^20): 16.5 / 16.6 / 16.5 / 16.9 / 17.1 ms).
+constexpr int kNLargeSieve = 21;  // (prime, value, root) items: 3 per prime
+constexpr int kNSievePrimes = kNSmallSieve + kNLargeSieve;
 constexpr uint32_t kSieveBits = 4096;            // odd d per chunk (512 bytes of LDS per wave)
 constexpr uint32_t kSieveMinSum = 1u << 15;
 constexpr uint32_t kSieveMinListed = 2048;  // segments shorter than this are not sieved
 constexpr uint32_t sieve_sqrt_offset(int i) { return i == 0 ? 0 : sieve_sqrt_offset(i - 1) + kSievePrimes[i - 1]; }
 constexpr uint32_t kSieveSqrtBytes = sieve_sqrt_offset(kNSievePrimes);
 static_assert(2 * 181 - 2 > 251, "sieve primes below the smallest probe value of a sum >= kSieveMinSum");
-static_assert(kNSievePrimes <= 43, "square roots and inverses are single bytes (primes below 255)");
+static_assert(kNSievePrimes <= 51, "square roots and inverses are single bytes (primes below 255)");
 struct SieveSqrtOffsets {
-    uint32_t v[43];
+    uint32_t v[51];
     constexpr SieveSqrtOffsets() : v()
     {
-        for (int i = 0; i < 43; ++i) v[i] = sieve_sqrt_offset(i);
+        for (int i = 0; i < 51; ++i) v[i] = sieve_sqrt_offset(i);
     }
 };
 __constant__ constexpr SieveSqrtOffsets kSieveSqrtOff{};
@@ -621,13 +630,15 @@ __device__ __forceinline__ void probe_batch(const SearchArgs &a, const uint2 *pq
     }
 }
 
-// The sieve over d of one sum.  Lane 3i+r (i < kNSievePrimes) owns the item
-// (prime q_i, value r): r = 0 the root of the linear value d + o*2^k - 2,
-// r = 1, 2 the roots (sm +- sqrt(D)) / 2^(k+1) of the quadratic
-// 2^k d^2 - s d - (2^k - 2) = 0 (2^k + d*o - 2 = 0 mod q) with
-// D = s^2 + 4 * 2^k (2^k - 2).  s mod q is taken once per sum, the root once
-// per k-segment; each chunk then only places it relative to the chunk's
-// first d.
+// The sieve over d of one sum.  An item is (prime q, value r): r = 0 the
+// root of the linear value d + o*2^k - 2, r = 1, 2 the roots
+// (sm +- sqrt(D)) / 2^(k+1) of the quadratic 2^k d^2 - s d - (2^k - 2) = 0
+// (2^k + d*o - 2 = 0 mod q) with D = s^2 + 4 * 2^k (2^k - 2).  Lane 3i+r
+// holds struck item (q_{kNSmallSieve+i}, r) in a SieveItem, and lanes below
+// 3 kNSmallSieve also hold masked item (q_i, r) packed in one register
+// (root | kind << 8 | s mod q << 16).  s mod q is taken once per sum, the
+// roots once per k-segment; each chunk then only places them relative to
+// the chunk's first d.
 struct SieveItem {
     uint32_t q = 1, half = 0;  // the item's prime; kSieveBits/2 mod q
     uint32_t sm = 0;           // s mod q
@@ -635,22 +646,22 @@ struct SieveItem {
     uint32_t kind = 0;         // 0: no root, 1: root, 2: every d (4^k = 1 and s 2^k = 2 mod q)
 };
 
-__device__ __forceinline__ void sieve_item_for_sum(SieveItem &it, uint32_t s, int lane)
+__device__ __forceinline__ void sieve_item_for_sum(SieveItem &it, uint32_t &small, uint32_t s, int lane)
 {
-    if (lane < 3 * kNSievePrimes) {
-        it.q = kSievePrimes[lane / 3];
+    if (lane < 3 * kNLargeSieve) {
+        it.q = kSievePrimes[kNSmallSieve + lane / 3];
         it.half = (kSieveBits / 2) % it.q;
         it.sm = s % it.q;
     }
+    small = lane < 3 * kNSmallSieve ? (s % kSievePrimes[lane / 3]) << 16 : 0u;
 }
 
-// (root, kind) of the lane's item for 2^k
+// (root, kind) of item (prime index pi, value which) for 2^k; (0, 0) unless valid
 __device__ __noinline__ uint2 sieve_item_for_k(uint32_t q, uint32_t sm, const uint32_t *sieveK,
-                                               const uint8_t *sieveSqrt, uint32_t k, int lane)
+                                               const uint8_t *sieveSqrt, uint32_t k, int pi, int which, bool valid)
 {
     uint2 it = make_uint2(0u, 0u);
-    if (lane >= 3 * kNSievePrimes) return it;
-    const int pi = lane / 3, which = lane - 3 * pi;
+    if (!valid) return it;
     const uint32_t rec = sieveK[pi * 32 + k];
     const uint32_t pk = rec & 255, iY = (rec >> 8) & 255, iX = (rec >> 16) & 255;
     if (which == 0) {
@@ -665,19 +676,57 @@ __device__ __noinline__ uint2 sieve_item_for_k(uint32_t q, uint32_t sm, const ui
     return it;
 }
 
+// Bits 0, q, 2q, ... of a 64-bit word.
+constexpr uint64_t every_qth_bit(uint32_t q)
+{
+    uint64_t m = 0;
+    for (uint32_t b = 0; b < 64; b += q) m |= 1ull << b;
+    return m;
+}
+
 // Marks, in bits (kSieveBits, bit b = odd d = dLo + 2b), the d of the
 // k-segment's splits whose 2^k + d*o - 2 or d + o*2^k - 2 has a sieve prime
-// factor (o = s - d*2^k).  Items of small primes mark many more bits than
-// those of large ones, so lane l marks the first half of its own item's
-// bits and the second half of item 62-l's: every lane then carries a small
-// and a large prime.
+// factor (o = s - d*2^k).  Lane l first writes bits 64l..64l+63 as the
+// union of the masked items' residue classes (the items' first bits taken
+// from the lanes that hold them, one readlane each), then the struck items
+// are OR-ed in: items of small primes mark many more bits than those of
+// large ones, so lane l strikes the first half of its own item's bits and
+// the second half of item 62-l's, and every lane carries a small and a
+// large prime.
 using lds_u32 = __attribute__((address_space(3))) uint32_t;
-__device__ __noinline__ void sieve_chunk(const SieveItem it, uint32_t dLo, lds_u32 *bits, int lane)
+__device__ __noinline__ void sieve_chunk(const SieveItem it, uint32_t small, uint32_t dLo, lds_u32 *bits, int lane)
 {
-    static_assert(3 * kNSievePrimes <= 63, "items fit lanes 0..62, paired l <-> 62-l");
+    static_assert(3 * kNLargeSieve <= 63, "struck items fit lanes 0..62, paired l <-> 62-l");
+    static_assert(kSieveBits == 64 * 64, "one 64-bit word of masked items per lane");
     constexpr uint32_t kHalf = kSieveBits / 2;
-    for (uint32_t i = (uint32_t)lane; i < kSieveBits / 32; i += 64) bits[i] = 0;
-    // own item: first bit b0 < q with dLo + 2 b0 = root (mod q), odd d
+    {
+        // masked item of this lane: its first bit b0 < q (odd d, dLo odd)
+        uint32_t place = 0;  // b0 | kind << 8
+        if (lane < 3 * kNSmallSieve) {
+            const uint32_t q = kSievePrimes[lane / 3], kind = (small >> 8) & 3;
+            uint32_t b0 = 0;
+            if (kind == 1) {
+                uint32_t t = (small & 255) + q - dLo % q;
+                if (t >= q) t -= q;
+                if (t & 1) t += q;
+                b0 = t >> 1;
+            }
+            place = b0 | kind << 8;
+        }
+        uint64_t m = 0;
+#pragma unroll
+        for (int i = 0; i < 3 * kNSmallSieve; ++i) {
+            const uint32_t q = kSievePrimes[i / 3];  // a constant once unrolled
+            const uint32_t v = __builtin_amdgcn_readlane(place, i);  // uniform
+            const uint32_t r = (64u * (uint32_t)lane) % q;           // first bit of this lane's word, mod q
+            const uint32_t u = (v & 255) - r, ph = min(u, u + q);   // (b0 - r) mod q
+            m |= ((v >> 8) == 1 ? every_qth_bit(q) : 0ull) << ph;
+        }
+        if (__ballot((place >> 8) == 2)) m = ~0ull;  // an item that marks every d (rare)
+        bits[2 * lane] = (uint32_t)m;
+        bits[2 * lane + 1] = (uint32_t)(m >> 32);
+    }
+    // own struck item: first bit b0 < q with dLo + 2 b0 = root (mod q), odd d
     uint32_t b0 = 0;
     if (it.kind == 1) {
         uint32_t t = it.root + it.q - dLo % it.q;  // d - dLo = t (mod q), t in [1, 2q)
@@ -821,13 +870,23 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
         FFX_COUNT(0, 1);
         FFX_COUNT(9, total);
     }
+    // splits visited, by the sum's outcome: 10 one split (a solution), 11 an
+    // early exit at two, 12 none
+    uint32_t visitedSplits = 0;
+#define FFX_COUNT_EXIT(i) \
+    do {                  \
+        if (lane == 0) FFX_COUNT(i, visitedSplits); \
+    } while (0)
+#else
+#define FFX_COUNT_EXIT(i) ((void)0)
 #endif
     // the chunk of the d sieve: segment sj, odd d in [sLo, sLo + 2 kSieveBits)
     int sj = -1;
     uint32_t sLo = 0;
     const bool sieveSum = s >= kSieveMinSum;
     SieveItem item;
-    if (sieveSum) sieve_item_for_sum(item, s, lane);
+    uint32_t small = 0;
+    if (sieveSum) sieve_item_for_sum(item, small, s, lane);
     for (uint32_t base = 0; base < total; base += 64 * kRoundSplits) {
         bool useSieve = false;
         if (sieveSum && base >= a.eager) {
@@ -847,11 +906,17 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
                 const uint32_t d0 = d_at(base), d1 = d_at(gl);
                 if (j0 != sj || d0 < sLo || d1 >= sLo + 2 * kSieveBits) {
                     if (j0 != sj) {
-                        const uint2 rk = sieve_item_for_k(item.q, item.sm, a.sieveK, a.sieveSqrt, e.y & 31, lane);
+                        const uint32_t k = e.y & 31;
+                        const uint2 rk = sieve_item_for_k(item.q, item.sm, a.sieveK, a.sieveSqrt, k,
+                                                          kNSmallSieve + lane / 3, lane % 3, lane < 3 * kNLargeSieve);
                         item.root = rk.x;
                         item.kind = rk.y;
+                        const bool masked = lane < 3 * kNSmallSieve;
+                        const uint2 rm = sieve_item_for_k(masked ? kSievePrimes[lane / 3] : 1u, small >> 16, a.sieveK,
+                                                          a.sieveSqrt, k, lane / 3, lane % 3, masked);
+                        small = (small & 0xFFFF0000u) | rm.y << 8 | rm.x;
                     }
-                    sieve_chunk(item, d0, (lds_u32 *)dbits, lane);
+                    sieve_chunk(item, small, d0, (lds_u32 *)dbits, lane);
                     sj = j0;
                     sLo = d0;
                 }
@@ -860,10 +925,14 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
         }
 #ifdef FFX_PROBE_COUNTERS
         if (lane == 0) FFX_COUNT(1, total - base < 64u * kRoundSplits ? total - base : 64u * kRoundSplits);
+        visitedSplits += total - base < 64u * kRoundSplits ? total - base : 64u * kRoundSplits;
+        if (lane == 0) FFX_COUNT(13, 1);
+        if (lane == 0 && useSieve) FFX_COUNT(14, 1);
 #endif
         // Split g of segment j, through the filters that need no map read;
         // survivors go to the probe queue.
-        auto filter = [&](uint32_t g, int j, uint4 e) {
+        auto filter = [&](uint32_t g, int j, uint4 e, auto sievedTag) {
+            constexpr bool sieved = decltype(sievedTag)::value;
             const bool live = g < total;
             const uint32_t k = e.y & 31, c = (e.y >> 8) & 15, ii = live ? g - e.x + (e.y >> 16) : 0u;
             const uint32_t q = c == 1 ? ii : __umulhi(ii, e.w);
@@ -873,13 +942,22 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
             // Power2Even without 3 | d*o: o must be prime (Power2Odd lists
             // hold only d with 3 !| d*o); fixed by d's class
             const bool needO = cls >> 7;
-            bool rx, ry, ro;
-            probe_residues(tab, d, o, k, e.z, rx, ry, ro);
-            ry = ry || d == o;
-            bool keep = live && rx && ry && (!needO || ro);
-            if (useSieve && keep && d != o) {
-                const uint32_t b = (d - sLo) >> 1;
-                keep = !((dbits[b >> 5] >> (b & 31)) & 1u);
+            bool keep = live;
+            if (!sieved) {
+                bool rx, ry, ro;
+                probe_residues(tab, d, o, k, e.z, rx, ry, ro);
+                keep = keep && rx && (ry || d == o) && (!needO || ro);
+            } else {
+                // the chunk holds X and Y for the primes 7..131 (the d = o
+                // split, whose Y is not tested, is decided by residues), so
+                // residues are taken only for that split and for a prime o
+                const uint32_t b = live ? (d - sLo) >> 1 : 0u;
+                keep = keep && (d == o || !((dbits[b >> 5] >> (b & 31)) & 1u));
+                if (keep && (needO || d == o)) {
+                    bool rx, ry, ro;
+                    probe_residues(tab, d, o, k, e.z, rx, ry, ro);
+                    keep = rx && (ry || d == o) && (!needO || ro);
+                }
             }
             const uint64_t km = __ballot(keep);
             if (keep) pq[pn + __popcll(km & lt)] = make_uint2(d | (k << kDBits), o | (needO ? kNeedO : 0u));
@@ -895,8 +973,13 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
         const uint32_t glast = (base + 64 * kRoundSplits < total ? base + 64 * kRoundSplits : total) - 1;
         if (jr + 1 >= nseg || glast < segs[jr + 1].x) {
             const uint4 e = segs[jr];
+            if (useSieve) {  // uniform; a sieved round lies in one segment
 #pragma unroll
-            for (int u = 0; u < kRoundSplits; ++u) filter(base + 64u * u + lane, jr, e);
+                for (int u = 0; u < kRoundSplits; ++u) filter(base + 64u * u + lane, jr, e, std::true_type{});
+            } else {
+#pragma unroll
+                for (int u = 0; u < kRoundSplits; ++u) filter(base + 64u * u + lane, jr, e, std::false_type{});
+            }
             hint = jr;
         } else {
             int segOf[kRoundSplits];
@@ -907,7 +990,7 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
                 if (g < total)
                     while (j + 1 < nseg && g >= segs[j + 1].x) ++j;
                 segOf[u] = j;
-                filter(g, j, segs[j]);
+                filter(g, j, segs[j], std::false_type{});
             }
             hint = __shfl(segOf[kRoundSplits - 1], 63);
         }
@@ -923,7 +1006,10 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
                 const int na = qn < 64 ? qn : 64;
                 drain(a, tab, queue, qn - na, na, lane, pr, cnt, te, to);
                 qn -= na;
-                if (cnt > 1) return 0;
+                if (cnt > 1) {
+                    FFX_COUNT_EXIT(11);
+                    return 0;
+                }
             }
             wave_sync_lds();
         }
@@ -933,9 +1019,17 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
         const int nb = qn < 64 ? qn : 64;
         drain(a, tab, queue, qn - nb, nb, lane, pr, cnt, te, to);
         qn -= nb;
-        if (cnt > 1) return 0;
+        if (cnt > 1) {
+            FFX_COUNT_EXIT(11);
+            return 0;
+        }
     }
-    if (cnt != 1) return 0;
+    if (cnt != 1) {
+        FFX_COUNT_EXIT(12);
+        return 0;
+    }
+    FFX_COUNT_EXIT(10);
+#undef FFX_COUNT_EXIT
 
     const uint32_t lo = te < to ? te : to, hi = te < to ? to : te;
     uint32_t slot = lo;
