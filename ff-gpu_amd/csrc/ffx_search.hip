@@ -27,9 +27,10 @@
 // lists only the classes of d mod 30 that can hold a valid split (the
 // Power2Odd family, both probe values coprime to 15).  Every listed split
 // first goes through the filters that need no map read: the Power2Even
-// family test, a residue filter (a probe value with a prime factor in 7..23
-// is composite; the residues of both probe values come from those of d, o
-// and 2^k) and, in long k-segments, a sieve over d by the primes 37..131.
+// family test and either, in long k-segments, one bit of a sieve over d by
+// the primes 7..167, or a residue filter (a probe value with a prime factor
+// in 7..23 is composite; the residues of both probe values come from those
+// of d, o and 2^k).
 // The few survivors are compacted into a per-wave LDS probe queue and probed
 // 128 at a time, so the random map reads run on full waves; splits that pass
 // both PTP probes go to a second queue for AllButOne, resolved 64 at a time,
@@ -114,27 +115,28 @@ constexpr uint32_t kTabWords = tab_offset(kNMods);
 // of kSievePrimes; splits on marked d are dropped before any map read, with
 // no per-split residue arithmetic (the split classes mod 30 already exclude
 // 3 and 5).  Both values exceed 2*sqrt(s-1) - 2 > 251 for every split of a
-// sum s >= kSieveMinSum, so a marked value is composite.  The primes 7..31
+// sum s >= kSieveMinSum, so a marked value is composite.  The primes 7..61
 // mark many bits each and are laid in as 64-bit word masks, every lane
-// building its own 64 bits from all their roots; the primes from 37 strike
+// building its own 64 bits from all their roots; the primes from 67 strike
 // their bits one lane per (prime, value, root).  The chunk's bits live in
 // 512 bytes of LDS per wave.
 constexpr uint32_t kSievePrimes[51] = {7,   11,  13,  17,  19,  23,  29,  31,  37,  41,  43,  47,  53,
                                        59,  61,  67,  71,  73,  79,  83,  89,  97,  101, 103, 107, 109,
                                        113, 127, 131, 137, 139, 149, 151, 157, 163, 167, 173, 179, 181,
                                        191, 193, 197, 199, 211, 223, 227, 229, 233, 239, 241, 251};
-constexpr int kNSmallSieve = 8;   // 7..31: word masks
-static_assert(kSievePrimes[kNSmallSieve - 1] < 64 && kSievePrimes[kNSmallSieve] > 31, "masked primes below 64, struck from 37");
-// Struck primes in use: 21 (37..131) measured as fast as 32 or 43 and faster
-// than 14 or 8 (odd sums [5, 2^20): 16.5 / 16.6 / 16.5 / 16.9 / 17.1 ms).
-constexpr int kNLargeSieve = 21;  // (prime, value, root) items: 3 per prime
+// Masked 7..61 and struck 67..167 measured fastest (odd sums [5, 2^21)
+// search: 33.2 ms, against 34.2 with 7..43 / 47..139, 34.9 with 7..31 /
+// 37..131 and 34.5 with 7..61 / 67..131).
+constexpr int kNSmallSieve = 15;  // 7..61: word masks
+static_assert(kSievePrimes[kNSmallSieve - 1] < 64 && kSievePrimes[kNSmallSieve] > 64, "masked primes below 64");
+constexpr int kNLargeSieve = 21;  // 67..167 struck: (prime, value, root) items, 3 per prime
 constexpr int kNSievePrimes = kNSmallSieve + kNLargeSieve;
 constexpr uint32_t kSieveBits = 4096;            // odd d per chunk (512 bytes of LDS per wave)
 constexpr uint32_t kSieveMinSum = 1u << 15;
 constexpr uint32_t kSieveMinListed = 2048;  // segments shorter than this are not sieved
 constexpr uint32_t sieve_sqrt_offset(int i) { return i == 0 ? 0 : sieve_sqrt_offset(i - 1) + kSievePrimes[i - 1]; }
 constexpr uint32_t kSieveSqrtBytes = sieve_sqrt_offset(kNSievePrimes);
-static_assert(2 * 181 - 2 > 251, "sieve primes below the smallest probe value of a sum >= kSieveMinSum");
+static_assert(2 * 181 - 2 > kSievePrimes[kNSievePrimes - 1], "sieve primes below the smallest probe value of a sum >= kSieveMinSum");
 static_assert(kNSievePrimes <= 51, "square roots and inverses are single bytes (primes below 255)");
 struct SieveSqrtOffsets {
     uint32_t v[51];
@@ -948,7 +950,7 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
                 probe_residues(tab, d, o, k, e.z, rx, ry, ro);
                 keep = keep && rx && (ry || d == o) && (!needO || ro);
             } else {
-                // the chunk holds X and Y for the primes 7..131 (the d = o
+                // the chunk holds X and Y for the primes 7..167 (the d = o
                 // split, whose Y is not tested, is decided by residues), so
                 // residues are taken only for that split and for a prime o
                 const uint32_t b = live ? (d - sLo) >> 1 : 0u;
