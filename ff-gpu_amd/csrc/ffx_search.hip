@@ -45,7 +45,6 @@
 #include <cstring>
 #include <mutex>
 #include <new>
-#include <type_traits>
 #include <vector>
 
 #include "ffx_internal.h"
@@ -933,8 +932,7 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
 #endif
         // Split g of segment j, through the filters that need no map read;
         // survivors go to the probe queue.
-        auto filter = [&](uint32_t g, int j, uint4 e, auto sievedTag) {
-            constexpr bool sieved = decltype(sievedTag)::value;
+        auto filter = [&](uint32_t g, int j, uint4 e) {
             const bool live = g < total;
             const uint32_t k = e.y & 31, c = (e.y >> 8) & 15, ii = live ? g - e.x + (e.y >> 16) : 0u;
             const uint32_t q = c == 1 ? ii : __umulhi(ii, e.w);
@@ -944,23 +942,9 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
             // Power2Even without 3 | d*o: o must be prime (Power2Odd lists
             // hold only d with 3 !| d*o); fixed by d's class
             const bool needO = cls >> 7;
-            bool keep = live;
-            if (!sieved) {
-                bool rx, ry, ro;
-                probe_residues(tab, d, o, k, e.z, rx, ry, ro);
-                keep = keep && rx && (ry || d == o) && (!needO || ro);
-            } else {
-                // the chunk holds X and Y for the primes 7..167 (the d = o
-                // split, whose Y is not tested, is decided by residues), so
-                // residues are taken only for that split and for a prime o
-                const uint32_t b = live ? (d - sLo) >> 1 : 0u;
-                keep = keep && (d == o || !((dbits[b >> 5] >> (b & 31)) & 1u));
-                if (keep && (needO || d == o)) {
-                    bool rx, ry, ro;
-                    probe_residues(tab, d, o, k, e.z, rx, ry, ro);
-                    keep = rx && (ry || d == o) && (!needO || ro);
-                }
-            }
+            bool rx, ry, ro;
+            probe_residues(tab, d, o, k, e.z, rx, ry, ro);
+            const bool keep = live && rx && (ry || d == o) && (!needO || ro);
             const uint64_t km = __ballot(keep);
             if (keep) pq[pn + __popcll(km & lt)] = make_uint2(d | (k << kDBits), o | (needO ? kNeedO : 0u));
 #ifdef FFX_PROBE_COUNTERS
@@ -975,12 +959,66 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
         const uint32_t glast = (base + 64 * kRoundSplits < total ? base + 64 * kRoundSplits : total) - 1;
         if (jr + 1 >= nseg || glast < segs[jr + 1].x) {
             const uint4 e = segs[jr];
-            if (useSieve) {  // uniform; a sieved round lies in one segment
+            if (useSieve) {
+                // a sieved round lies in one segment: its fields are uniform,
+                // and the chunk holds X and Y for the primes 7..167, so a
+                // split costs two LDS reads (its class, its chunk word), all
+                // of a lane's splits read together; residues are taken only
+                // for the d = o split (its Y is not tested) and for a prime o
+                const uint32_t k = __builtin_amdgcn_readfirstlane(e.y) & 31;
+                const uint32_t c = (__builtin_amdgcn_readfirstlane(e.y) >> 8) & 15;
+                const uint32_t first = __builtin_amdgcn_readfirstlane(e.x) - (__builtin_amdgcn_readfirstlane(e.y) >> 16);
+                const uint32_t w = __builtin_amdgcn_readfirstlane(e.w);
+                uint32_t d[kRoundSplits], cl[kRoundSplits], wd[kRoundSplits];
+                bool live[kRoundSplits];
 #pragma unroll
-                for (int u = 0; u < kRoundSplits; ++u) filter(base + 64u * u + lane, jr, e, std::true_type{});
+                for (int u = 0; u < kRoundSplits; ++u) {
+                    const uint32_t g = base + 64u * u + lane;
+                    live[u] = g < total;
+                    const uint32_t ii = live[u] ? g - first : 0u;
+                    const uint32_t q = c == 1 ? ii : __umulhi(ii, w);
+                    d[u] = q;
+                    cl[u] = segOffs[jr * 16 + (ii - q * c)];
+                }
+#pragma unroll
+                for (int u = 0; u < kRoundSplits; ++u) {
+                    d[u] = __umul24(d[u], 30u) + (cl[u] & 0x7Fu);
+                    const uint32_t b = live[u] ? (d[u] - sLo) >> 1 : 0u;
+                    wd[u] = dbits[b >> 5] >> (b & 31);
+                }
+                bool keep[kRoundSplits], slow = false;
+#pragma unroll
+                for (int u = 0; u < kRoundSplits; ++u) {
+                    const uint32_t o = s - (d[u] << k);
+                    keep[u] = live[u] && (d[u] == o || !(wd[u] & 1u));
+                    slow = slow || (keep[u] && ((cl[u] >> 7) || d[u] == o));
+                }
+                if (__ballot(slow)) {  // rare: sums 3p hold the prime-o classes
+#pragma unroll
+                    for (int u = 0; u < kRoundSplits; ++u) {
+                        const uint32_t o = s - (d[u] << k);
+                        const bool needO = cl[u] >> 7;
+                        if (keep[u] && (needO || d[u] == o)) {
+                            bool rx, ry, ro;
+                            probe_residues(tab, d[u], o, k, e.z, rx, ry, ro);
+                            keep[u] = rx && (ry || d[u] == o) && (!needO || ro);
+                        }
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < kRoundSplits; ++u) {
+                    const uint64_t km = __ballot(keep[u]);
+                    if (keep[u])
+                        pq[pn + __popcll(km & lt)] =
+                            make_uint2(d[u] | (k << kDBits), (s - (d[u] << k)) | ((cl[u] >> 7) ? kNeedO : 0u));
+#ifdef FFX_PROBE_COUNTERS
+                    if (lane == 0) FFX_COUNT(2, __popcll(km));
+#endif
+                    pn += __popcll(km);
+                }
             } else {
 #pragma unroll
-                for (int u = 0; u < kRoundSplits; ++u) filter(base + 64u * u + lane, jr, e, std::false_type{});
+                for (int u = 0; u < kRoundSplits; ++u) filter(base + 64u * u + lane, jr, e);
             }
             hint = jr;
         } else {
@@ -992,7 +1030,7 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
                 if (g < total)
                     while (j + 1 < nseg && g >= segs[j + 1].x) ++j;
                 segOf[u] = j;
-                filter(g, j, segs[j], std::false_type{});
+                filter(g, j, segs[j]);
             }
             hint = __shfl(segOf[kRoundSplits - 1], 63);
         }
