@@ -639,20 +639,24 @@ __device__ __forceinline__ void probe_batch(const SearchArgs &a, const uint2 *pq
 // 3 kNSmallSieve also hold masked item (q_i, r) packed in one register
 // (root | kind << 8 | s mod q << 16).  s mod q is taken once per sum, the
 // roots once per k-segment; each chunk then only places them relative to
-// the chunk's first d.
+// the chunk's first d.  Fields are bytes, packed two registers to a lane
+// (the search runs at its register limit).
 struct SieveItem {
-    uint32_t q = 1, half = 0;  // the item's prime; kSieveBits/2 mod q
-    uint32_t sm = 0;           // s mod q
-    uint32_t root = 0;         // d = root (mod q) makes the item's value divisible by q
-    uint32_t kind = 0;         // 0: no root, 1: root, 2: every d (4^k = 1 and s 2^k = 2 mod q)
+    uint32_t qhs = 1;  // q | (kSieveBits/2 mod q) << 8 | (s mod q) << 16
+    uint32_t rk = 0;   // root | kind << 8: d = root (mod q) makes the value divisible by q;
+                       // kind 0 no root, 1 root, 2 every d (4^k = 1 and s 2^k = 2 mod q)
+    __device__ uint32_t q() const { return qhs & 255; }
+    __device__ uint32_t half() const { return (qhs >> 8) & 255; }
+    __device__ uint32_t sm() const { return qhs >> 16; }
+    __device__ uint32_t root() const { return rk & 255; }
+    __device__ uint32_t kind() const { return rk >> 8; }
 };
 
 __device__ __forceinline__ void sieve_item_for_sum(SieveItem &it, uint32_t &small, uint32_t s, int lane)
 {
     if (lane < 3 * kNLargeSieve) {
-        it.q = kSievePrimes[kNSmallSieve + lane / 3];
-        it.half = (kSieveBits / 2) % it.q;
-        it.sm = s % it.q;
+        const uint32_t q = kSievePrimes[kNSmallSieve + lane / 3];
+        it.qhs = q | ((kSieveBits / 2) % q) << 8 | (s % q) << 16;
     }
     small = lane < 3 * kNSmallSieve ? (s % kSievePrimes[lane / 3]) << 16 : 0u;
 }
@@ -729,23 +733,25 @@ __device__ __noinline__ void sieve_chunk(const SieveItem it, uint32_t small, uin
     }
     // own struck item: first bit b0 < q with dLo + 2 b0 = root (mod q), odd d
     uint32_t b0 = 0;
-    if (it.kind == 1) {
-        uint32_t t = it.root + it.q - dLo % it.q;  // d - dLo = t (mod q), t in [1, 2q)
-        if (t >= it.q) t -= it.q;
-        if (t & 1) t += it.q;  // odd d (dLo is odd): an even offset
+    const uint32_t q = it.q(), kind = it.kind();
+    if (kind == 1) {
+        uint32_t t = it.root() + q - dLo % q;  // d - dLo = t (mod q), t in [1, 2q)
+        if (t >= q) t -= q;
+        if (t & 1) t += q;  // odd d (dLo is odd): an even offset
         b0 = t >> 1;
     }
     // the partner item's second half
     const int partner = 62 - lane;
-    const uint32_t qB = __shfl(it.q, partner), hB = __shfl(it.half, partner), kindB = __shfl(it.kind, partner);
+    const uint32_t qhsB = __shfl(it.qhs, partner), qB = qhsB & 255, hB = (qhsB >> 8) & 255,
+                   kindB = __shfl(it.rk, partner) >> 8;
     const uint32_t b0B = __shfl(b0, partner);
     wave_sync_lds();
     if (lane < 63) {
-        if (it.kind == 2) {
+        if (kind == 2) {
             for (uint32_t i = 0; i < kHalf / 32; ++i)
                 __hip_atomic_fetch_or(&bits[i], 0xFFFFFFFFu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
-        } else if (it.kind == 1) {
-            for (uint32_t b = b0; b < kHalf; b += it.q)
+        } else if (kind == 1) {
+            for (uint32_t b = b0; b < kHalf; b += q)
                 __hip_atomic_fetch_or(&bits[b >> 5], 1u << (b & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
         }
         if (kindB == 2) {
@@ -881,54 +887,155 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
 #else
 #define FFX_COUNT_EXIT(i) ((void)0)
 #endif
-    // the chunk of the d sieve: segment sj, odd d in [sLo, sLo + 2 kSieveBits)
+    // the d sieve's items, their roots taken for segment sj
     int sj = -1;
-    uint32_t sLo = 0;
     const bool sieveSum = s >= kSieveMinSum;
     SieveItem item;
     uint32_t small = 0;
     if (sieveSum) sieve_item_for_sum(item, small, s, lane);
-    for (uint32_t base = 0; base < total; base += 64 * kRoundSplits) {
-        bool useSieve = false;
-        if (sieveSum && base >= a.eager) {
-            // uniform: the round's first and last split, their segment and d
-            const uint32_t gl = (base + 64 * kRoundSplits < total ? base + 64 * kRoundSplits : total) - 1;
-            int j0 = hint;
-            while (j0 + 1 < nseg && base >= segs[j0 + 1].x) ++j0;
-            const uint32_t segEnd = j0 + 1 < nseg ? segs[j0 + 1].x : total;
-            if (gl < segEnd && segEnd - segs[j0].x >= kSieveMinListed) {
-                const uint4 e = segs[j0];
-                const uint32_t c = (e.y >> 8) & 15;
-                auto d_at = [&](uint32_t g) {
-                    const uint32_t ii = g - e.x + (e.y >> 16);
-                    const uint32_t q = c == 1 ? ii : __umulhi(ii, e.w);
-                    return 30 * q + (segOffs[j0 * 16 + (ii - q * c)] & 0x7Fu);
-                };
-                const uint32_t d0 = d_at(base), d1 = d_at(gl);
-                if (j0 != sj || d0 < sLo || d1 >= sLo + 2 * kSieveBits) {
-                    if (j0 != sj) {
-                        const uint32_t k = e.y & 31;
-                        const uint2 rk = sieve_item_for_k(item.q, item.sm, a.sieveK, a.sieveSqrt, k,
-                                                          kNSmallSieve + lane / 3, lane % 3, lane < 3 * kNLargeSieve);
-                        item.root = rk.x;
-                        item.kind = rk.y;
-                        const bool masked = lane < 3 * kNSmallSieve;
-                        const uint2 rm = sieve_item_for_k(masked ? kSievePrimes[lane / 3] : 1u, small >> 16, a.sieveK,
-                                                          a.sieveSqrt, k, lane / 3, lane % 3, masked);
-                        small = (small & 0xFFFF0000u) | rm.y << 8 | rm.x;
-                    }
-                    sieve_chunk(item, small, d0, (lds_u32 *)dbits, lane);
-                    sj = j0;
-                    sLo = d0;
-                }
-                useSieve = true;
+    // Probes queued splits until fewer than 128 wait (all of them when
+    // flush), draining the AllButOne queue in between; true once the sum
+    // has two valid splits.
+    auto process = [&](bool flush) -> bool {
+        if (pn < 128 && !(flush && pn > 0)) return false;
+        wave_sync_lds();
+        while (pn >= 128 || (flush && pn > 0)) {
+            const int nb = pn < 128 ? pn : 128;
+            probe_batch(a, pq, pn - nb, nb, lane, lt, queue, qn);
+            pn -= nb;
+            wave_sync_lds();
+            while (qn >= 64 || (flush && qn > 0)) {
+                const int na = qn < 64 ? qn : 64;
+                drain(a, tab, queue, qn - na, na, lane, pr, cnt, te, to);
+                qn -= na;
+                if (cnt > 1) return true;
             }
+            wave_sync_lds();
+        }
+        return false;
+    };
+    uint32_t base = 0;
+    while (base < total) {
+        // uniform: the segment of base
+        int jr = hint;
+        while (jr + 1 < nseg && base >= segs[jr + 1].x) ++jr;
+        const uint32_t segEnd = jr + 1 < nseg ? segs[jr + 1].x : total;
+        if (sieveSum && base >= a.eager && segEnd - segs[jr].x >= kSieveMinListed) {
+            // Scan: the rest of a long segment, [base, segEnd), is read off
+            // the sieve over d instead of split by split.  Chunk by chunk
+            // (odd d in [lo, lo + 2 kSieveBits)), lane l takes words l and
+            // 64 + l of the chunk: its survivors are the unmarked bits of
+            // listed classes within [d(base), d(segEnd - 1)], plus the
+            // d = o split if listed (the chunk does not decide it: its Y is
+            // not tested).  Survivors go straight to the probe queue, each
+            // lane writing its own at its place in the wave's count.
+            const uint32_t ey = __builtin_amdgcn_readfirstlane(segs[jr].y);
+            const uint32_t k = ey & 31, c = (ey >> 8) & 15;
+            const uint32_t first = __builtin_amdgcn_readfirstlane(segs[jr].x) - (ey >> 16);
+            const uint32_t w = __builtin_amdgcn_readfirstlane(segs[jr].w);
+            auto d_at = [&](uint32_t g) {
+                const uint32_t ii = g - first;
+                const uint32_t q = c == 1 ? ii : __umulhi(ii, w);
+                return __umul24(q, 30u) + (segOffs[jr * 16 + (ii - q * c)] & 0x7Fu);
+            };
+            const uint32_t dStart = __builtin_amdgcn_readfirstlane(d_at(base));
+            const uint32_t dEnd = __builtin_amdgcn_readfirstlane(d_at(segEnd - 1));
+            // listed classes t (d = 2t+1 mod 30), and those whose o must be prime
+            uint32_t cm = 0, nm = 0;
+            if ((uint32_t)lane < c) {
+                const uint32_t v = segOffs[jr * 16 + lane];
+                cm = 1u << ((v & 0x7Fu) >> 1);
+                nm = (v >> 7) ? cm : 0u;
+            }
+            for (int off = 1; off < 16; off <<= 1) {
+                cm |= __shfl_xor(cm, off);
+                nm |= __shfl_xor(nm, off);
+            }
+            const uint32_t classes = __builtin_amdgcn_readfirstlane(cm), needOs = __builtin_amdgcn_readfirstlane(nm);
+            const uint64_t rep = classes | (uint64_t)classes << 15 | (uint64_t)classes << 30 | (uint64_t)classes << 45;
+            const uint32_t m2 = (1u << k) + 1, q2 = s / m2;
+            const uint32_t dStar = (q2 * m2 == s && (q2 & 1)) ? q2 : 0u;  // the d = o split, odd
+            if (jr != sj) {
+                const uint2 rk = sieve_item_for_k(item.q(), item.sm(), a.sieveK, a.sieveSqrt, k,
+                                                  kNSmallSieve + lane / 3, lane % 3, lane < 3 * kNLargeSieve);
+                item.rk = rk.x | rk.y << 8;
+                const bool masked = lane < 3 * kNSmallSieve;
+                const uint2 rm = sieve_item_for_k(masked ? kSievePrimes[lane / 3] : 1u, small >> 16, a.sieveK,
+                                                  a.sieveSqrt, k, lane / 3, lane % 3, masked);
+                small = (small & 0xFFFF0000u) | rm.y << 8 | rm.x;
+                sj = jr;
+            }
+#ifdef FFX_PROBE_COUNTERS
+            if (lane == 0) FFX_COUNT(1, segEnd - base);
+            visitedSplits += segEnd - base;
+#endif
+            for (uint32_t lo = dStart; lo <= dEnd; lo += 2 * kSieveBits) {
+                sieve_chunk(item, small, lo, (lds_u32 *)dbits, lane);
+                const uint32_t t0 = (lo % 30) >> 1;  // class of bit 0
+                const uint32_t bEnd = (dEnd - lo) >> 1;
+                const uint32_t bStar = (dStar - lo) >> 1;  // huge unless the d = o split is in the chunk
+#pragma unroll 1
+                for (uint32_t h = 0; h < kSieveBits / 2048; ++h) {
+                    // queued in windows that fit the probe queue; a word's
+                    // survivors are read again after each probe pass rather
+                    // than held across it (registers)
+                    for (uint32_t done = 0;;) {
+                        const uint32_t wi = 64 * h + lane, wb = 32 * wi, r = (t0 + 2 * wi) % 15;  // 32 = 2 mod 15
+                        uint32_t sv = ~dbits[wi];
+                        if (dStar >= lo && (bStar >> 5) == wi) sv |= 1u << (bStar & 31);
+                        sv &= (uint32_t)(rep >> r);
+                        sv &= wb > bEnd ? 0u : (bEnd - wb >= 31 ? ~0u : (2u << (bEnd - wb)) - 1u);
+                        // this lane's place among the wave's survivors (counts <= 32: six ballots)
+                        const uint32_t n = __popc(sv);
+                        uint32_t pre = 0, tot = 0;
+#pragma unroll
+                        for (int j = 0; j < 6; ++j) {
+                            const uint64_t bj = __ballot((n >> j) & 1u);
+                            pre += (uint32_t)__popcll(bj & lt) << j;
+                            tot += (uint32_t)__popcll(bj) << j;
+                        }
+#ifdef FFX_PROBE_COUNTERS
+                        if (lane == 0 && done == 0) {
+                            FFX_COUNT(2, tot);
+                            FFX_COUNT(13, 1);
+                            FFX_COUNT(14, 1);
+                        }
+#endif
+                        const uint32_t take = min(tot - done, (uint32_t)(kProbeCap - pn));
+                        uint32_t idx = pre;
+                        while (sv) {
+                            const uint32_t bit = __ffs(sv) - 1;
+                            sv &= sv - 1;
+                            if (idx - done < take) {
+                                const uint32_t d = lo + 2 * (wb + bit), o = s - (d << k);
+                                const bool needO = needOs && ((needOs >> ((r + bit) % 15)) & 1u);
+                                pq[pn + (idx - done)] = make_uint2(d | (k << kDBits), o | (needO ? kNeedO : 0u));
+                            }
+                            ++idx;
+                        }
+                        pn += take;
+                        done += take;
+                        const bool more = done < tot;
+                        if (process(false)) {
+                            FFX_COUNT_EXIT(11);
+                            return 0;
+                        }
+                        if (!more) break;
+                    }
+                }
+            }
+            hint = jr;
+            base = segEnd;
+            if (process(base >= total)) {
+                FFX_COUNT_EXIT(11);
+                return 0;
+            }
+            continue;
         }
 #ifdef FFX_PROBE_COUNTERS
         if (lane == 0) FFX_COUNT(1, total - base < 64u * kRoundSplits ? total - base : 64u * kRoundSplits);
         visitedSplits += total - base < 64u * kRoundSplits ? total - base : 64u * kRoundSplits;
         if (lane == 0) FFX_COUNT(13, 1);
-        if (lane == 0 && useSieve) FFX_COUNT(14, 1);
 #endif
         // Split g of segment j, through the filters that need no map read;
         // survivors go to the probe queue.
@@ -952,74 +1059,13 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
 #endif
             pn += __popcll(km);
         };
-        // uniform: the segment of the round's first split, and whether the
-        // whole round lies in it (then no lane searches the segment table)
-        int jr = hint;
-        while (jr + 1 < nseg && base >= segs[jr + 1].x) ++jr;
+        // uniform: whether the whole round lies in the segment of base (then
+        // no lane searches the segment table)
         const uint32_t glast = (base + 64 * kRoundSplits < total ? base + 64 * kRoundSplits : total) - 1;
-        if (jr + 1 >= nseg || glast < segs[jr + 1].x) {
+        if (glast < segEnd) {
             const uint4 e = segs[jr];
-            if (useSieve) {
-                // a sieved round lies in one segment: its fields are uniform,
-                // and the chunk holds X and Y for the primes 7..167, so a
-                // split costs two LDS reads (its class, its chunk word), all
-                // of a lane's splits read together; residues are taken only
-                // for the d = o split (its Y is not tested) and for a prime o
-                const uint32_t k = __builtin_amdgcn_readfirstlane(e.y) & 31;
-                const uint32_t c = (__builtin_amdgcn_readfirstlane(e.y) >> 8) & 15;
-                const uint32_t first = __builtin_amdgcn_readfirstlane(e.x) - (__builtin_amdgcn_readfirstlane(e.y) >> 16);
-                const uint32_t w = __builtin_amdgcn_readfirstlane(e.w);
-                uint32_t d[kRoundSplits], cl[kRoundSplits], wd[kRoundSplits];
-                bool live[kRoundSplits];
 #pragma unroll
-                for (int u = 0; u < kRoundSplits; ++u) {
-                    const uint32_t g = base + 64u * u + lane;
-                    live[u] = g < total;
-                    const uint32_t ii = live[u] ? g - first : 0u;
-                    const uint32_t q = c == 1 ? ii : __umulhi(ii, w);
-                    d[u] = q;
-                    cl[u] = segOffs[jr * 16 + (ii - q * c)];
-                }
-#pragma unroll
-                for (int u = 0; u < kRoundSplits; ++u) {
-                    d[u] = __umul24(d[u], 30u) + (cl[u] & 0x7Fu);
-                    const uint32_t b = live[u] ? (d[u] - sLo) >> 1 : 0u;
-                    wd[u] = dbits[b >> 5] >> (b & 31);
-                }
-                bool keep[kRoundSplits], slow = false;
-#pragma unroll
-                for (int u = 0; u < kRoundSplits; ++u) {
-                    const uint32_t o = s - (d[u] << k);
-                    keep[u] = live[u] && (d[u] == o || !(wd[u] & 1u));
-                    slow = slow || (keep[u] && ((cl[u] >> 7) || d[u] == o));
-                }
-                if (__ballot(slow)) {  // rare: sums 3p hold the prime-o classes
-#pragma unroll
-                    for (int u = 0; u < kRoundSplits; ++u) {
-                        const uint32_t o = s - (d[u] << k);
-                        const bool needO = cl[u] >> 7;
-                        if (keep[u] && (needO || d[u] == o)) {
-                            bool rx, ry, ro;
-                            probe_residues(tab, d[u], o, k, e.z, rx, ry, ro);
-                            keep[u] = rx && (ry || d[u] == o) && (!needO || ro);
-                        }
-                    }
-                }
-#pragma unroll
-                for (int u = 0; u < kRoundSplits; ++u) {
-                    const uint64_t km = __ballot(keep[u]);
-                    if (keep[u])
-                        pq[pn + __popcll(km & lt)] =
-                            make_uint2(d[u] | (k << kDBits), (s - (d[u] << k)) | ((cl[u] >> 7) ? kNeedO : 0u));
-#ifdef FFX_PROBE_COUNTERS
-                    if (lane == 0) FFX_COUNT(2, __popcll(km));
-#endif
-                    pn += __popcll(km);
-                }
-            } else {
-#pragma unroll
-                for (int u = 0; u < kRoundSplits; ++u) filter(base + 64u * u + lane, jr, e);
-            }
+            for (int u = 0; u < kRoundSplits; ++u) filter(base + 64u * u + lane, jr, e);
             hint = jr;
         } else {
             int segOf[kRoundSplits];
@@ -1035,23 +1081,10 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
             hint = __shfl(segOf[kRoundSplits - 1], 63);
         }
         const bool flush = base + 64 * kRoundSplits <= a.eager || base + 64 * kRoundSplits >= total;
-        if (pn < 128 && !(flush && pn > 0)) continue;
-        wave_sync_lds();
-        while (pn >= 128 || (flush && pn > 0)) {
-            const int nb = pn < 128 ? pn : 128;
-            probe_batch(a, pq, pn - nb, nb, lane, lt, queue, qn);
-            pn -= nb;
-            wave_sync_lds();
-            while (qn >= 64 || (flush && qn > 0)) {
-                const int na = qn < 64 ? qn : 64;
-                drain(a, tab, queue, qn - na, na, lane, pr, cnt, te, to);
-                qn -= na;
-                if (cnt > 1) {
-                    FFX_COUNT_EXIT(11);
-                    return 0;
-                }
-            }
-            wave_sync_lds();
+        base += 64 * kRoundSplits;
+        if (process(flush)) {
+            FFX_COUNT_EXIT(11);
+            return 0;
         }
     }
     wave_sync_lds();
