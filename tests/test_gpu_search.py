@@ -147,3 +147,82 @@ def test_search_rejects_bad_arguments(c1_map):
     out = ffxpy.DeviceBuffer(64)
     assert lib.ffx_search_slots_async(plan.h, dmap.ptr, M, 5, 5001, out.ptr, None) == -34  # above the plan
     assert lib.ffx_search_slots_async(plan.h, dmap.ptr, 7, 5, 99, out.ptr, None) == -22    # map too small
+
+
+def _listed_splits(s, is_prime):
+    """The splits the kernel lists for sum s, enumerated d by d: for each
+    k from the largest with 3*2^k <= s-3 down to 2, every odd d in
+    [dmin, (s-3) >> k] whose class mod 30 the family and the probe values'
+    residues mod 3 and 5 admit (the d = o split's class kept whole).
+    None when statement 1 or two Power2 primes end the sum first."""
+    if is_prime[s - 2] or (s % 3 == 0 and not is_prime[s // 3]):
+        return None
+    if sum(1 for k in range(2, 32) if (1 << k) < s - 2 and is_prime[s - (1 << k)]) > 1:
+        return None
+    total = 0
+    k = ((s - 3) // 3).bit_length() - 1
+    while k >= 2:
+        D = (s - 3) >> k
+        dmin = 5 if k & 1 else 3
+        d = np.arange(dmin | 1, D + 1, 2, dtype=np.int64)
+        o = s - (d << k)
+        x, y = (1 << k) + d * o - 2, d + o * (1 << k) - 2
+        ok = np.ones(len(d), dtype=bool)
+        if k & 1:
+            ok &= (d % 3 != 0) & (o % 3 != 0)
+        # the rule is per class of d mod 30 (o, x, y mod 15 follow from it);
+        # the d = o split's class keeps its y unchecked
+        dstar = s // ((1 << k) + 1) if s % ((1 << k) + 1) == 0 else -1
+        star_cls = (dstar % 30) if dstar >= dmin and dstar % 2 == 1 and dstar <= D else -1
+        ok &= (x % 3 != 0) & (x % 5 != 0) & (((y % 3 != 0) & (y % 5 != 0)) | (d % 30 == star_cls))
+        total += int(ok.sum())
+        k -= 1
+    return total
+
+
+def test_listed_splits_are_exactly_the_splits_with_d_up_to_its_bound():
+    # the count of listed splits per segment is P + (offset <= rem) per class
+    # with P = D / 30; counting (D + 1) / 30 periods listed a whole extra
+    # period, d beyond (s - 3) >> k, whenever 30 divides D + 1.  The counters
+    # build sums the kernel's listed splits over a launch; here they equal
+    # the splits enumerated d by d, over sums where 30 | D + 1 for some k.
+    import ctypes as C
+    import os
+    lim = 1 << 17
+    sieve = np.ones(lim, dtype=bool)
+    sieve[:2] = False
+    for p in range(2, int(lim ** 0.5) + 1):
+        if sieve[p]:
+            sieve[p * p::p] = False
+    lo, hi = 32769, 36767  # odd sums from 2^15 + 1: long k-segments, scanned
+    want = 0
+    hits = 0
+    for s in range(lo, hi + 1, 2):
+        n = _listed_splits(s, sieve)
+        if n is not None:
+            want += n
+            hits += any(((s - 3) >> k) % 30 == 29 for k in range(2, 16))
+    assert hits >= 50  # 53 sums reach the splits with 30 | D + 1 for some k
+    lib = C.CDLL(os.path.join(os.path.dirname(ffxpy.LIB_PATH), "libffx_counters.so"))
+    lib.ffx_init.argtypes = [C.c_int]
+    lib.ffx_probe_counters.argtypes = [C.c_void_p, C.c_int]
+    lib.ffx_search_plan_create.argtypes = [C.POINTER(C.c_void_p), C.c_uint32]
+    lib.ffx_search_plan_destroy.argtypes = [C.c_void_p]
+    lib.ffx_search_slots_async.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_void_p,
+                                           C.c_void_p]
+    assert lib.ffx_init(0) == 0
+    M = 268435471
+    dmap = ffxpy.sieve_map(M)
+    plan = C.c_void_p()
+    assert lib.ffx_search_plan_create(C.byref(plan), hi) == 0
+    slots = ffxpy.DeviceBuffer(((hi - lo) // 2 + 1) * 4)
+    counts = np.zeros(16, dtype=np.uint64)
+    assert lib.ffx_probe_counters(counts.ctypes.data, 1) == 0
+    say("listed splits of odd sums", lo, hi)
+    assert lib.ffx_search_slots_async(plan, dmap.ptr, M, lo, hi, slots.ptr, None) == 0
+    assert lib.ffx_device_sync() == 0
+    assert lib.ffx_probe_counters(counts.ctypes.data, 1) == 0
+    lib.ffx_search_plan_destroy(plan)
+    slots.close()
+    dmap.close()
+    assert int(counts[9]) == want
