@@ -133,18 +133,11 @@ constexpr int kNSievePrimes = kNSmallSieve + kNLargeSieve;
 constexpr uint32_t kSieveBits = 4096;            // odd d per chunk (512 bytes of LDS per wave)
 constexpr uint32_t kSieveMinSum = 1u << 15;
 constexpr uint32_t kSieveMinListed = 512;  // segments shorter than this are not sieved (256..1024 measured alike, 2048 0.4 % slower)
-constexpr uint32_t sieve_sqrt_offset(int i) { return i == 0 ? 0 : sieve_sqrt_offset(i - 1) + kSievePrimes[i - 1]; }
-constexpr uint32_t kSieveSqrtBytes = sieve_sqrt_offset(kNSievePrimes);
+// square roots mod each sieve prime: 256 bytes per prime (an address from
+// the prime's index and the residue alone, no offset table to read first)
+constexpr uint32_t kSieveSqrtBytes = kNSievePrimes * 256;
 static_assert(2 * 181 - 2 > kSievePrimes[kNSievePrimes - 1], "sieve primes below the smallest probe value of a sum >= kSieveMinSum");
 static_assert(kNSievePrimes <= 51, "square roots and inverses are single bytes (primes below 255)");
-struct SieveSqrtOffsets {
-    uint32_t v[51];
-    constexpr SieveSqrtOffsets() : v()
-    {
-        for (int i = 0; i < 51; ++i) v[i] = sieve_sqrt_offset(i);
-    }
-};
-__constant__ constexpr SieveSqrtOffsets kSieveSqrtOff{};
 
 struct SearchArgs {
     const uint8_t *map;
@@ -637,7 +630,7 @@ __device__ __forceinline__ void probe_batch(const SearchArgs &a, const uint2 *pq
 // (2^k + d*o - 2 = 0 mod q) with D = s^2 + 4 * 2^k (2^k - 2).  Lane 3i+r
 // holds struck item (q_{kNSmallSieve+i}, r) in a SieveItem, and lanes below
 // 3 kNSmallSieve also hold masked item (q_i, r) packed in one register
-// (root | kind << 8 | s mod q << 16).  s mod q is taken once per sum, the
+// (root | kind << 8 | s mod q << 16 | q << 24).  s mod q is taken once per sum, the
 // roots once per k-segment; each chunk then only places them relative to
 // the chunk's first d.  Fields are bytes, packed two registers to a lane
 // (the search runs at its register limit).
@@ -658,27 +651,51 @@ __device__ __forceinline__ void sieve_item_for_sum(SieveItem &it, uint32_t &smal
         const uint32_t q = kSievePrimes[kNSmallSieve + lane / 3];
         it.qhs = q | ((kSieveBits / 2) % q) << 8 | (s % q) << 16;
     }
-    small = lane < 3 * kNSmallSieve ? (s % kSievePrimes[lane / 3]) << 16 : 0u;
+    if (lane < 3 * kNSmallSieve) {
+        const uint32_t q = kSievePrimes[lane / 3];
+        small = (s % q) << 16 | q << 24;
+    } else {
+        small = 1u << 24;  // q = 1: no item
+    }
 }
 
-// (root, kind) of item (prime index pi, value which) for 2^k; (0, 0) unless valid
-__device__ __noinline__ uint2 sieve_item_for_k(uint32_t q, uint32_t sm, const uint32_t *sieveK,
-                                               const uint8_t *sieveSqrt, uint32_t k, int pi, int which, bool valid)
+// (root, kind) of an item for 2^k = pk (mod q), from its table record and,
+// for the quadratic's roots, the square root r of its discriminant
+__device__ __forceinline__ uint2 item_root(uint32_t q, uint32_t sm, uint32_t pk, uint32_t rec, uint32_t r, int which)
 {
     uint2 it = make_uint2(0u, 0u);
-    if (!valid) return it;
-    const uint32_t rec = sieveK[pi * 32 + k];
-    const uint32_t pk = rec & 255, iY = (rec >> 8) & 255, iX = (rec >> 16) & 255;
+    const uint32_t iY = (rec >> 8) & 255, iX = (rec >> 16) & 255;
     if (which == 0) {
         const uint32_t rhs = (2 + q - (sm * pk) % q) % q;  // d (1 - 4^k) = 2 - s 2^k
         if (rec >> 24) it = make_uint2((rhs * iY) % q, 1u);
         else if (rhs == 0) it.y = 2;  // 4^k = 1 mod q: the value is s 2^k - 2 for every d
-    } else {
-        const uint32_t D = (sm * sm + 4 * pk * ((pk + q - 2) % q)) % q;
-        const uint32_t r = sieveSqrt[kSieveSqrtOff.v[pi] + D];
-        if (r != 0xFFu) it = make_uint2((((which == 1 ? sm + r : sm + q - r) % q) * iX) % q, 1u);
+    } else if (r != 0xFFu) {
+        it = make_uint2((((which == 1 ? sm + r : sm + q - r) % q) * iX) % q, 1u);
     }
     return it;
+}
+
+// Roots of both of the lane's items for 2^k: x for its struck item (prime
+// q, s mod q = sm), y for its masked item (packed in small, q = 1 when the
+// lane has none); root | kind << 8 each.  2^k mod q is computed, so every
+// table read (record, square root) is issued before any is used: one memory
+// latency per k-segment instead of four.
+__device__ __noinline__ uint2 sieve_items_for_k(uint32_t q, uint32_t sm, uint32_t small, const uint32_t *sieveK,
+                                                const uint8_t *sieveSqrt, uint32_t k, int lane)
+{
+    const int which = lane % 3;
+    const bool vS = lane < 3 * kNLargeSieve, vM = lane < 3 * kNSmallSieve;
+    const int piS = kNSmallSieve + lane / 3, piM = lane / 3;
+    const uint32_t qM = small >> 24, smM = (small >> 16) & 255;
+    const uint32_t pkS = (1u << k) % q, pkM = (1u << k) % qM;
+    const uint32_t DS = (sm * sm + 4 * pkS * ((pkS + q - 2) % q)) % q;
+    const uint32_t DM = (smM * smM + 4 * pkM * ((pkM + qM - 2) % qM)) % qM;
+    const uint32_t recS = vS ? sieveK[piS * 32 + k] : 0u, recM = vM ? sieveK[piM * 32 + k] : 0u;
+    const uint32_t rS = vS && which ? sieveSqrt[piS * 256 + DS] : 0xFFu;
+    const uint32_t rM = vM && which ? sieveSqrt[piM * 256 + DM] : 0xFFu;
+    const uint2 a = vS ? item_root(q, sm, pkS, recS, rS, which) : make_uint2(0u, 0u);
+    const uint2 b = vM ? item_root(qM, smM, pkM, recM, rM, which) : make_uint2(0u, 0u);
+    return make_uint2(a.x | a.y << 8, b.x | b.y << 8);
 }
 
 // Bits 0, q, 2q, ... of a 64-bit word.
@@ -708,7 +725,7 @@ __device__ __noinline__ void sieve_chunk(const SieveItem it, uint32_t small, uin
         // masked item of this lane: its first bit b0 < q (odd d, dLo odd)
         uint32_t place = 0;  // b0 | kind << 8
         if (lane < 3 * kNSmallSieve) {
-            const uint32_t q = kSievePrimes[lane / 3], kind = (small >> 8) & 3;
+            const uint32_t q = small >> 24, kind = (small >> 8) & 3;
             uint32_t b0 = 0;
             if (kind == 1) {
                 uint32_t t = (small & 255) + q - dLo % q;
@@ -958,13 +975,9 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
             const uint32_t m2 = (1u << k) + 1, q2 = s / m2;
             const uint32_t dStar = (q2 * m2 == s && (q2 & 1)) ? q2 : 0u;  // the d = o split, odd
             if (jr != sj) {
-                const uint2 rk = sieve_item_for_k(item.q(), item.sm(), a.sieveK, a.sieveSqrt, k,
-                                                  kNSmallSieve + lane / 3, lane % 3, lane < 3 * kNLargeSieve);
-                item.rk = rk.x | rk.y << 8;
-                const bool masked = lane < 3 * kNSmallSieve;
-                const uint2 rm = sieve_item_for_k(masked ? kSievePrimes[lane / 3] : 1u, small >> 16, a.sieveK,
-                                                  a.sieveSqrt, k, lane / 3, lane % 3, masked);
-                small = (small & 0xFFFF0000u) | rm.y << 8 | rm.x;
+                const uint2 rk = sieve_items_for_k(item.q(), item.sm(), small, a.sieveK, a.sieveSqrt, k, lane);
+                item.rk = rk.x;
+                small = (small & 0xFFFF0000u) | rk.y;
                 sj = jr;
             }
 #ifdef FFX_PROBE_COUNTERS
@@ -1333,7 +1346,7 @@ static SearchTables build_search_tables(uint32_t sumLimitMax)
             pk = 2 * pk % q;
         }
         for (uint32_t r = 0; r < q; ++r)
-            if (t.sieveSqrt[sieve_sqrt_offset(i) + r * r % q] == 0xFF) t.sieveSqrt[sieve_sqrt_offset(i) + r * r % q] = (uint8_t)r;
+            if (t.sieveSqrt[i * 256 + r * r % q] == 0xFF) t.sieveSqrt[i * 256 + r * r % q] = (uint8_t)r;
     }
     return t;
 }
