@@ -75,14 +75,15 @@ constexpr int kRoundSplits = FFX_ROUND_SPLITS;  // splits per lane per filter ro
 constexpr int kProbeCap = 128 + 64 * kRoundSplits;  // < 128 left after the batches + one round
 static_assert(kRoundSplits >= 1 && kRoundSplits <= 4, "1..4 splits per lane per round");
 constexpr uint32_t kNeedO = 1u << 31;  // probe entry: o must be prime too (Power2Even, 3 !| d*o)
-constexpr uint32_t kGrab = 2;      // default sums taken per atomic (the autotuned choice)
+constexpr uint32_t kGrab = 4;      // default sums taken per atomic (the autotuned choice)
 constexpr int kMaxPrimes = 12;     // distinct odd primes of q < 3^31 (q <= s^2/16, s < 2^26)
 constexpr uint32_t kDBits = 27;    // queue entry: d | k << 27
 constexpr int kMaxSegs = 32;
 // Default: while a sum has visited fewer splits than this, every queued
 // candidate is resolved after each step (most sums stop within a few hundred
 // splits); beyond it candidates are resolved 64 at a time.
-constexpr uint32_t kEagerSplits = 1024;  // the autotuned choice for odd sums to 2^20 and 2^21
+// Long k-segments are scanned off the sieve over d only past this point too.
+constexpr uint32_t kEagerSplits = 256;  // the autotuned choice for odd sums to 2^20 and 2^21
 constexpr uint32_t kBlocksPerCU = 4;  // default persistent grid: CUs x this (all resident)
 
 // Residue filter: a probe value that shares a factor with one of the moduli

@@ -40,10 +40,11 @@ namespace {
 // Larger maps, where most primes strike a segment at most a few times and
 // each segment pays every prime's first-multiple computation, are faster
 // with the largest segment the LDS holds beside the patterns, one
-// workgroup per CU: 16 GiB map 39.5 ms with 112 KiB against 40.1 (96 KiB),
-// 41.2 (80 KiB) and 41.8 (64 KiB, two per CU).
+// workgroup resident per CU: 16 GiB map 39.5 ms with 112 KiB against 40.1
+// (96 KiB), 41.2 (80 KiB) and 41.8 (64 KiB, two per CU).  The grid holds two
+// workgroups per CU (two waves of shorter runs, 0.5 % faster than one).
 constexpr uint32_t kSmallMapSegBytes = 32768, kSmallMapBlocksPerCU = 4;
-constexpr uint32_t kLargeMapSegBytes = 114688, kLargeMapBlocksPerCU = 1;
+constexpr uint32_t kLargeMapSegBytes = 114688, kLargeMapBlocksPerCU = 2;
 constexpr uint64_t kLargeMapFrom = 1ull << 38;
 constexpr int kThreads = 1024;
 constexpr int kWaves = kThreads / 64;
