@@ -47,6 +47,15 @@ constexpr uint32_t kSmallMapSegBytes = 32768, kSmallMapBlocksPerCU = 4;
 constexpr uint32_t kLargeMapSegBytes = 114688, kLargeMapBlocksPerCU = 2;
 constexpr uint64_t kLargeMapFrom = 1ull << 38;
 constexpr int kThreads = 1024;
+// Cooperative primes up to segOdds / kWheelDiv strike through a mod-6 wheel
+// (a third fewer LDS atomics; above it a lane has too few strikes per
+// progression to pay for the second pass).  16 GiB map: 39.3 ms without the
+// wheel, 36.5 / 35.6 / 35.4 / 35.6 / 36.0 ms with divisors 96 / 192 / 384 /
+// 768 / 1536; 70 GB map 198 -> 177 ms.
+#ifndef FFX_SIEVE_WHEEL_DIV
+#define FFX_SIEVE_WHEEL_DIV 384
+#endif
+constexpr uint32_t kWheelDiv = FFX_SIEVE_WHEEL_DIV;
 constexpr int kWaves = kThreads / 64;
 
 constexpr int kGroups = 8;
@@ -97,6 +106,7 @@ struct SieveArgs {
     const double *invp;      // 1.0 / primes[i]
     uint32_t nprimes;
     uint32_t ncoop;          // primes[0, ncoop) strike wave-cooperatively (> 64 strikes per segment)
+    uint32_t nwheel;         // primes[0, nwheel) of those skip their multiples of 3 (mod-6 wheel)
     uint32_t ncarry;         // lane primes carried per thread (<= the kernel's kCarry)
     const uint32_t *patterns;  // kPatWords words, groups back to back
     uint64_t segFirst, segCount;
@@ -264,7 +274,72 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kSegBy
             const uint32_t half = (uint32_t)lane >> 5, hl = (uint32_t)lane & 31;
             for (uint32_t pb = 2 * (uint32_t)wave; pb < a.ncoop; pb += 2 * kWaves) {
                 const uint32_t pi = pb + half;
-                if (pi < a.ncoop) {
+                if (pi < a.ncoop && pi < a.nwheel) {
+                    // Mod-6 wheel: the odd multiples m*p with 3 | m are
+                    // already struck (3 is a pattern prime), so only
+                    // m = 1 and m = 5 (mod 6) are struck, as two progressions
+                    // of odd-index stride 3p; lane l of the half strikes
+                    // elements l + 32t of each, a stride of 96p odd values =
+                    // 3p whole words, so its bit stays put.  coopNext keeps
+                    // the first wheel multiple at or past the segment's start
+                    // as offset << 1 | (m = 5 mod 6).
+                    const uint32_t p = a.primes[pi];
+                    uint32_t first, five;
+                    if (si == runLo) {
+                        uint64_t f = first_strike(j0, p, a.invp[pi]);  // < (p*p-1)/2 < 2^32
+                        // its multiple m (exact: 2(j0+f)+1 < 2^53 is m*p)
+                        const uint64_t m = (uint64_t)rint((2.0 * (double)(j0 + f) + 1.0) * a.invp[pi]);
+                        uint32_t r6 = (uint32_t)(m % 6);
+                        if (r6 == 3) {  // m + 2 = 5 (mod 6): one multiple on
+                            f += p;
+                            r6 = 5;
+                        }
+                        first = (uint32_t)f;
+                        five = r6 == 5;
+                    } else {
+                        const uint32_t v = coopNext[pi];
+                        first = v >> 1;
+                        five = v & 1u;
+                    }
+                    const uint32_t p3 = 3 * p;  // < 2^16
+                    const uint32_t stepBytes = 4u * p3, endBytes = ((limit + 31) >> 5) << 2;
+                    char *segBytes = reinterpret_cast<char *>(seg);
+#pragma unroll
+                    for (int r = 0; r < 2; ++r) {
+                        // the progression of first (r = 0), then the other
+                        // residue: from m = 1 the next m = 5 is m + 4
+                        // (index + 2p), from m = 5 the next m = 1 is m + 2
+                        const uint32_t fr = r == 0 ? first : first + (five ? p : 2 * p);
+                        const uint32_t fiveR = r == 0 ? five : 1u - five;
+                        const uint32_t jr = fr + __umul24(hl, p3);
+                        uint32_t je = jr;
+                        if (jr < limit) {
+                            const uint32_t mask = 1u << (jr & 31);
+                            uint32_t ad = (jr >> 5) << 2;
+                            if (kSegBytes == 32768) {
+                                const uint32_t endPair = endBytes > stepBytes ? endBytes - stepBytes : 0u;
+                                for (; ad < endPair; ad += 2 * stepBytes) {
+                                    atomicOr(reinterpret_cast<uint32_t *>(segBytes + ad), mask);
+                                    atomicOr(reinterpret_cast<uint32_t *>(segBytes + ad + stepBytes), mask);
+                                }
+                            }
+                            for (; ad < endBytes; ad += stepBytes)
+                                atomicOr(reinterpret_cast<uint32_t *>(segBytes + ad), mask);
+                            je = (ad << 3) | (jr & 31);
+                        }
+                        // the 32 lanes end on 32 consecutive elements at or
+                        // past kSegOdds; the one ending below kSegOdds + 3p
+                        // (lane 0 when the progression did not strike) holds
+                        // its first in the next segment, and the smaller of
+                        // the two progressions' firsts is kept
+                        const uint32_t d = je - kSegOdds;
+                        if (limit == kSegOdds && (d < p3 || (hl == 0 && fr >= kSegOdds))) {
+                            const uint32_t v = d << 1 | fiveR;
+                            if (r == 0) coopNext[pi] = v;
+                            else atomicMin(&coopNext[pi], v);
+                        }
+                    }
+                } else if (pi < a.ncoop) {
                     const uint32_t p = a.primes[pi];
                     uint32_t first;
                     if (si == runLo) {
@@ -511,6 +586,9 @@ static int launch(const ffx_sieve_plan *pl, uint8_t *dst, uint64_t dstBase, uint
                          pl->hostPrimes.begin());
     const uint32_t coopCap = segBytes == kSmallMapSegBytes ? coop_capacity(kSmallMapSegBytes * 8) : coop_capacity(kLargeMapSegBytes * 8);
     if (a.ncoop > coopCap) return fail(FFX_EINVAL, "sieve: %u cooperative primes exceed the table", a.ncoop);
+    // the mod-6 wheel for the cooperative primes up to segOdds / kWheelDiv
+    a.nwheel = (uint32_t)(std::lower_bound(pl->hostPrimes.begin(), pl->hostPrimes.end(), (uint32_t)(segOdds / kWheelDiv)) -
+                          pl->hostPrimes.begin());
     a.patterns = pl->dPatterns;
     a.segFirst = byteLo / segBytes;
     a.segCount = (byteHi + segBytes - 1) / segBytes - a.segFirst;
