@@ -168,6 +168,9 @@ __device__ __forceinline__ uint32_t to_map_order(uint32_t v)
 #ifndef FFX_FILL_UNROLL
 #define FFX_FILL_UNROLL 4
 #endif
+// carried lane primes: the next strike's offset, bit 31 set when its
+// multiplier is 5 (mod 6) (offsets stay below 2^31: a run is shorter)
+constexpr uint32_t kFivePhase = 1u << 31;
 constexpr int kCarrySmall = 12;      // 32 KiB segments, primes re-read
 constexpr int kCarryHeld = 32;       // large segments, primes held
 constexpr int kCarryReread = 64;     // large segments, primes re-read
@@ -232,8 +235,13 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kSegBy
             if (kHoldPrimes) cp[kHoldPrimes ? k : 0] = 0;
             if ((uint32_t)k < a.ncarry && pi < a.nprimes) {
                 const uint32_t p = a.primes[pi];
-                const uint64_t f = first_strike(j0, p, a.invp[pi]);
-                cn[k] = (uint32_t)(f < runOdds ? f : runOdds);
+                uint64_t f = first_strike(j0, p, a.invp[pi]);
+                // mod-6 wheel: its multiple m (exact, as for the cooperative
+                // primes); a multiple of 3 moves on to m + 2 = 5 (mod 6)
+                const uint64_t m = (uint64_t)rint((2.0 * (double)(j0 + f) + 1.0) * a.invp[pi]);
+                const uint32_t r6 = (uint32_t)(m % 6);
+                if (r6 == 3) f += p;
+                cn[k] = (uint32_t)(f < runOdds ? f : runOdds) | (r6 != 1 ? kFivePhase : 0u);
                 if (kHoldPrimes) cp[kHoldPrimes ? k : 0] = p;
             }
         }
@@ -409,9 +417,16 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kSegBy
                     pnext = cpr[(k + 1) * kThreads];
                 }
                 if ((uint32_t)k < a.ncarry && lane0 + (uint32_t)k * kThreads < a.nprimes) {
-                    uint32_t j = cn[k];
-                    for (; j < limit; j += p) atomicOr(&seg[j >> 5], 1u << (j & 31));
-                    cn[k] = j - kSegOdds;
+                    // mod-6 wheel over the multiplier: from m = 1 (mod 6) the
+                    // next m = 5 is 2p odd indices on, from m = 5 the next
+                    // m = 1 is p on; the odd multiples of 3 are the pattern's
+                    uint32_t j = cn[k] & ~kFivePhase, st = (cn[k] & kFivePhase) ? p : 2 * p;
+                    const uint32_t p3 = 3 * p;
+                    for (; j < limit; st = p3 - st) {
+                        atomicOr(&seg[j >> 5], 1u << (j & 31));
+                        j += st;
+                    }
+                    cn[k] = (j - kSegOdds) | (st == p ? kFivePhase : 0u);
                 }
             }
             // primes beyond the carried ones: first strike from j0 in
