@@ -17,11 +17,13 @@
 //    the reference's MSB-first layout (bit position b^7 within the word).
 //  * Primes 3..61 never strike: their multiples are OR-ed in as periodic word
 //    patterns (eight groups, periods 105..3599 words) kept in LDS.
-//  * Primes 67..segOdds/64 (more than 64 strikes per segment) are struck
+//  * Primes 67..segOdds/192 (more than 192 strikes per segment) are struck
 //    cooperatively: two primes per wavefront, lane l of a half striking
 //    every 32nd multiple (a fixed bit, an advancing word).  Larger primes
 //    are struck one prime per lane, carrying their next strike across the
-//    workgroup's run of segments.  Strikes are LDS atomic ORs (ds_or_b32).
+//    workgroup's run of segments.  Both skip the odd multiples of 3 (a
+//    mod-6 wheel over the multiplier).  Strikes are LDS atomic ORs
+//    (ds_or_b32).
 //  * The segment leaves LDS through 16-byte coalesced stores.
 #include <algorithm>
 #include <cmath>
@@ -78,8 +80,16 @@ constexpr uint32_t pattern_words()
 }
 constexpr uint32_t kPatWords = pattern_words();  // 10238 words, 40 KiB
 
+// Primes with more than kCoopDiv strikes per segment strike cooperatively.
+// With the wheels, one per lane is cheaper from fewer strikes than it was
+// without them: 16 GiB map 34.1 / 32.3 / 31.5 / 30.9 / 31.2 / 31.8 / 33.2 ms
+// with 64 / 96 / 128 / 192 / 256 / 384 / 512 (70 GB map 170 -> 153 ms, 4 GiB
+// 8.1 -> 7.7 ms at 192).
+constexpr uint32_t kCoopDiv = 192;
+
 // Odd primes in (kLargestPatternPrime, x): the cooperative primes of a
-// segment of 64x odd values, counted at compile time to size their table.
+// segment of kCoopDiv * x odd values, counted at compile time to size their
+// table.
 constexpr uint32_t coop_primes_below(uint32_t x)
 {
     uint32_t n = 0;
@@ -95,7 +105,7 @@ constexpr uint32_t coop_primes_below(uint32_t x)
     return n;
 }
 // table entries for a segment of segOdds odd values, rounded up to 64
-constexpr uint32_t coop_capacity(uint32_t segOdds) { return (coop_primes_below(segOdds / 64) + 63) & ~63u; }
+constexpr uint32_t coop_capacity(uint32_t segOdds) { return (coop_primes_below(segOdds / kCoopDiv) + 63) & ~63u; }
 
 struct SieveArgs {
     uint8_t *dst;          // receives map bytes [byteLo, byteHi) at dst[byte - dstBase]
@@ -105,7 +115,7 @@ struct SieveArgs {
     const uint32_t *primes;  // sieving primes > 61, ascending
     const double *invp;      // 1.0 / primes[i]
     uint32_t nprimes;
-    uint32_t ncoop;          // primes[0, ncoop) strike wave-cooperatively (> 64 strikes per segment)
+    uint32_t ncoop;          // primes[0, ncoop) strike wave-cooperatively (> kCoopDiv strikes per segment)
     uint32_t nwheel;         // primes[0, nwheel) of those skip their multiples of 3 (mod-6 wheel)
     uint32_t ncarry;         // lane primes carried per thread (<= the kernel's kCarry)
     const uint32_t *patterns;  // kPatWords words, groups back to back
@@ -596,8 +606,8 @@ static int launch(const ffx_sieve_plan *pl, uint8_t *dst, uint64_t dstBase, uint
     a.invp = pl->dInv;
     a.nprimes = pl->nprimes;
     const uint64_t segBytes = pl->segBytes, segOdds = segBytes * 8;
-    // primes with more than 64 strikes per segment strike wave-cooperatively
-    a.ncoop = (uint32_t)(std::lower_bound(pl->hostPrimes.begin(), pl->hostPrimes.end(), (uint32_t)(segOdds / 64)) -
+    // primes with more than kCoopDiv strikes per segment strike wave-cooperatively
+    a.ncoop = (uint32_t)(std::lower_bound(pl->hostPrimes.begin(), pl->hostPrimes.end(), (uint32_t)(segOdds / kCoopDiv)) -
                          pl->hostPrimes.begin());
     const uint32_t coopCap = segBytes == kSmallMapSegBytes ? coop_capacity(kSmallMapSegBytes * 8) : coop_capacity(kLargeMapSegBytes * 8);
     if (a.ncoop > coopCap) return fail(FFX_EINVAL, "sieve: %u cooperative primes exceed the table", a.ncoop);
