@@ -28,7 +28,7 @@
 // Power2Odd family, both probe values coprime to 15).  Every listed split
 // first goes through the filters that need no map read: the Power2Even
 // family test and either, in long k-segments, one bit of a sieve over d by
-// the primes 7..167, or a residue filter (a probe value with a prime factor
+// the primes 7..251, or a residue filter (a probe value with a prime factor
 // in 7..23 is composite; the residues of both probe values come from those
 // of d, o and 2^k).
 // The few survivors are compacted into a per-wave LDS probe queue and probed
@@ -66,10 +66,12 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / 64;
 constexpr int kQueueCap = 192;     // < 64 left after a drain + 128 from one probe batch
-// Four splits per lane per filter round (measured: 26.4 ms against 28.8 with
-// two at the same occupancy, 36 with one).
+// Two splits per lane per filter round: with long segments scanned off the
+// sieve over d, the filter rounds are a small share and two measure as fast
+// as four (odd sums to 2^21: 27.81 ms both), with a smaller probe queue (four
+// won when every split went through the rounds: 26.4 against 28.8 ms).
 #ifndef FFX_ROUND_SPLITS
-#define FFX_ROUND_SPLITS 4
+#define FFX_ROUND_SPLITS 2
 #endif
 constexpr int kRoundSplits = FFX_ROUND_SPLITS;  // splits per lane per filter round
 constexpr int kProbeCap = 128 + 64 * kRoundSplits;  // < 128 left after the batches + one round
@@ -130,7 +132,9 @@ constexpr uint32_t kSievePrimes[51] = {7,   11,  13,  17,  19,  23,  29,  31,  3
 constexpr int kNSmallSieve = 15;  // 7..61: word masks
 static_assert(kSievePrimes[kNSmallSieve - 1] < 64 && kSievePrimes[kNSmallSieve] > 64, "masked primes below 64");
 constexpr int kNLargeSieve = 21;  // 67..167 struck: (prime, value, root) items, 3 per prime
-constexpr int kNSievePrimes = kNSmallSieve + kNLargeSieve;
+constexpr int kNFarSieve = 15;    // 173..251 struck, one item per lane (few bits each): odd sums to
+                                  // 2^21 27.8 -> 25.5 ms, map reads 1.75 G -> 1.53 G
+constexpr int kNSievePrimes = kNSmallSieve + kNLargeSieve + kNFarSieve;
 constexpr uint32_t kSieveBits = 4096;            // odd d per chunk (512 bytes of LDS per wave)
 constexpr uint32_t kSieveMinSum = 1u << 15;
 constexpr uint32_t kSieveMinListed = 512;  // segments shorter than this are not sieved (256..1024 measured alike, 2048 0.4 % slower)
@@ -646,8 +650,15 @@ struct SieveItem {
     __device__ uint32_t kind() const { return rk >> 8; }
 };
 
-__device__ __forceinline__ void sieve_item_for_sum(SieveItem &it, uint32_t &small, uint32_t s, int lane)
+__device__ __forceinline__ void sieve_item_for_sum(SieveItem &it, uint32_t &small, uint32_t &far, uint32_t s, int lane)
 {
+    // far item of lanes below 3 kNFarSieve: q | s mod q << 8 | root << 16 | kind << 24
+    if (lane < 3 * kNFarSieve) {
+        const uint32_t q = kSievePrimes[kNSmallSieve + kNLargeSieve + lane / 3];
+        far = q | (s % q) << 8;
+    } else {
+        far = 1u;
+    }
     if (lane < 3 * kNLargeSieve) {
         const uint32_t q = kSievePrimes[kNSmallSieve + lane / 3];
         it.qhs = q | ((kSieveBits / 2) % q) << 8 | (s % q) << 16;
@@ -681,22 +692,27 @@ __device__ __forceinline__ uint2 item_root(uint32_t q, uint32_t sm, uint32_t pk,
 // lane has none); root | kind << 8 each.  2^k mod q is computed, so every
 // table read (record, square root) is issued before any is used: one memory
 // latency per k-segment instead of four.
-__device__ __noinline__ uint2 sieve_items_for_k(uint32_t q, uint32_t sm, uint32_t small, const uint32_t *sieveK,
-                                                const uint8_t *sieveSqrt, uint32_t k, int lane)
+__device__ __noinline__ uint3 sieve_items_for_k(uint32_t q, uint32_t sm, uint32_t small, uint32_t far,
+                                                const uint32_t *sieveK, const uint8_t *sieveSqrt, uint32_t k, int lane)
 {
     const int which = lane % 3;
-    const bool vS = lane < 3 * kNLargeSieve, vM = lane < 3 * kNSmallSieve;
-    const int piS = kNSmallSieve + lane / 3, piM = lane / 3;
+    const bool vS = lane < 3 * kNLargeSieve, vM = lane < 3 * kNSmallSieve, vF = lane < 3 * kNFarSieve;
+    const int piS = kNSmallSieve + lane / 3, piM = lane / 3, piF = kNSmallSieve + kNLargeSieve + lane / 3;
     const uint32_t qM = small >> 24, smM = (small >> 16) & 255;
-    const uint32_t pkS = (1u << k) % q, pkM = (1u << k) % qM;
+    const uint32_t qF = far & 255, smF = (far >> 8) & 255;
+    const uint32_t pkS = (1u << k) % q, pkM = (1u << k) % qM, pkF = (1u << k) % qF;
     const uint32_t DS = (sm * sm + 4 * pkS * ((pkS + q - 2) % q)) % q;
     const uint32_t DM = (smM * smM + 4 * pkM * ((pkM + qM - 2) % qM)) % qM;
+    const uint32_t DF = (smF * smF + 4 * pkF * ((pkF + qF - 2) % qF)) % qF;
     const uint32_t recS = vS ? sieveK[piS * 32 + k] : 0u, recM = vM ? sieveK[piM * 32 + k] : 0u;
+    const uint32_t recF = vF ? sieveK[piF * 32 + k] : 0u;
     const uint32_t rS = vS && which ? sieveSqrt[piS * 256 + DS] : 0xFFu;
     const uint32_t rM = vM && which ? sieveSqrt[piM * 256 + DM] : 0xFFu;
+    const uint32_t rF = vF && which ? sieveSqrt[piF * 256 + DF] : 0xFFu;
     const uint2 a = vS ? item_root(q, sm, pkS, recS, rS, which) : make_uint2(0u, 0u);
     const uint2 b = vM ? item_root(qM, smM, pkM, recM, rM, which) : make_uint2(0u, 0u);
-    return make_uint2(a.x | a.y << 8, b.x | b.y << 8);
+    const uint2 c = vF ? item_root(qF, smF, pkF, recF, rF, which) : make_uint2(0u, 0u);
+    return make_uint3(a.x | a.y << 8, b.x | b.y << 8, c.x | c.y << 8);
 }
 
 // Bits 0, q, 2q, ... of a 64-bit word.
@@ -717,7 +733,8 @@ constexpr uint64_t every_qth_bit(uint32_t q)
 // the second half of item 62-l's, and every lane carries a small and a
 // large prime.
 using lds_u32 = __attribute__((address_space(3))) uint32_t;
-__device__ __noinline__ void sieve_chunk(const SieveItem it, uint32_t small, uint32_t dLo, lds_u32 *bits, int lane)
+__device__ __noinline__ void sieve_chunk(const SieveItem it, uint32_t small, uint32_t far, uint32_t dLo, lds_u32 *bits,
+                                        int lane)
 {
     static_assert(3 * kNLargeSieve <= 63, "struck items fit lanes 0..62, paired l <-> 62-l");
     static_assert(kSieveBits == 64 * 64, "one 64-bit word of masked items per lane");
@@ -781,6 +798,18 @@ __device__ __noinline__ void sieve_chunk(const SieveItem it, uint32_t small, uin
             for (; b < kSieveBits; b += qB)
                 __hip_atomic_fetch_or(&bits[b >> 5], 1u << (b & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
         }
+    }
+    // far item: all its bits (at most kSieveBits / 173 + 1 = 24)
+    const uint32_t qF = far & 255, kindF = far >> 24;
+    if (kindF == 2) {
+        for (uint32_t i = 0; i < kSieveBits / 32; ++i)
+            __hip_atomic_fetch_or(&bits[i], 0xFFFFFFFFu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    } else if (kindF == 1) {
+        uint32_t t = ((far >> 16) & 255) + qF - dLo % qF;
+        if (t >= qF) t -= qF;
+        if (t & 1) t += qF;
+        for (uint32_t b = t >> 1; b < kSieveBits; b += qF)
+            __hip_atomic_fetch_or(&bits[b >> 5], 1u << (b & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
     }
     wave_sync_lds();
 }
@@ -911,8 +940,8 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
     int sj = -1;
     const bool sieveSum = s >= kSieveMinSum;
     SieveItem item;
-    uint32_t small = 0;
-    if (sieveSum) sieve_item_for_sum(item, small, s, lane);
+    uint32_t small = 0, far = 1;
+    if (sieveSum) sieve_item_for_sum(item, small, far, s, lane);
     // Probes queued splits until fewer than 128 wait (all of them when
     // flush), draining the AllButOne queue in between; true once the sum
     // has two valid splits.
@@ -976,9 +1005,10 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
             const uint32_t m2 = (1u << k) + 1, q2 = s / m2;
             const uint32_t dStar = (q2 * m2 == s && (q2 & 1)) ? q2 : 0u;  // the d = o split, odd
             if (jr != sj) {
-                const uint2 rk = sieve_items_for_k(item.q(), item.sm(), small, a.sieveK, a.sieveSqrt, k, lane);
+                const uint3 rk = sieve_items_for_k(item.q(), item.sm(), small, far, a.sieveK, a.sieveSqrt, k, lane);
                 item.rk = rk.x;
                 small = (small & 0xFFFF0000u) | rk.y;
+                far = (far & 0xFFFFu) | rk.z << 16;
                 sj = jr;
             }
 #ifdef FFX_PROBE_COUNTERS
@@ -986,7 +1016,7 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
             visitedSplits += segEnd - base;
 #endif
             for (uint32_t lo = dStart; lo <= dEnd; lo += 2 * kSieveBits) {
-                sieve_chunk(item, small, lo, (lds_u32 *)dbits, lane);
+                sieve_chunk(item, small, far, lo, (lds_u32 *)dbits, lane);
                 const uint32_t t0 = (lo % 30) >> 1;  // class of bit 0
                 const uint32_t bEnd = (dEnd - lo) >> 1;
                 const uint32_t bStar = (dStar - lo) >> 1;  // huge unless the d = o split is in the chunk
