@@ -122,10 +122,18 @@ constexpr uint32_t kTabWords = tab_offset(kNMods);
 // building its own 64 bits from all their roots; the primes from 67 strike
 // their bits one lane per (prime, value, root).  The chunk's bits live in
 // 512 bytes of LDS per wave.
-constexpr uint32_t kSievePrimes[51] = {7,   11,  13,  17,  19,  23,  29,  31,  37,  41,  43,  47,  53,
-                                       59,  61,  67,  71,  73,  79,  83,  89,  97,  101, 103, 107, 109,
-                                       113, 127, 131, 137, 139, 149, 151, 157, 163, 167, 173, 179, 181,
-                                       191, 193, 197, 199, 211, 223, 227, 229, 233, 239, 241, 251};
+constexpr uint32_t kSievePrimes[135] = {
+       7,   11,   13,   17,   19,   23,   29,   31,   37,   41,   43,   47,   53,
+      59,   61,   67,   71,   73,   79,   83,   89,   97,  101,  103,  107,  109,
+     113,  127,  131,  137,  139,  149,  151,  157,  163,  167,  173,  179,  181,
+     191,  193,  197,  199,  211,  223,  227,  229,  233,  239,  241,  251,  257,
+     263,  269,  271,  277,  281,  283,  293,  307,  311,  313,  317,  331,  337,
+     347,  349,  353,  359,  367,  373,  379,  383,  389,  397,  401,  409,  419,
+     421,  431,  433,  439,  443,  449,  457,  461,  463,  467,  479,  487,  491,
+     499,  503,  509,  521,  523,  541,  547,  557,  563,  569,  571,  577,  587,
+     593,  599,  601,  607,  613,  617,  619,  631,  641,  643,  647,  653,  659,
+     661,  673,  677,  683,  691,  701,  709,  719,  727,  733,  739,  743,  751,
+     757,  761,  769,  773,  787};
 // Masked 7..61 and struck 67..167 measured fastest (odd sums [5, 2^21)
 // search: 33.2 ms, against 34.2 with 7..43 / 47..139, 34.9 with 7..31 /
 // 37..131 and 34.5 with 7..61 / 67..131).
@@ -134,15 +142,30 @@ static_assert(kSievePrimes[kNSmallSieve - 1] < 64 && kSievePrimes[kNSmallSieve] 
 constexpr int kNLargeSieve = 21;  // 67..167 struck: (prime, value, root) items, 3 per prime
 constexpr int kNFarSieve = 15;    // 173..251 struck, one item per lane (few bits each): odd sums to
                                   // 2^21 27.8 -> 25.5 ms, map reads 1.75 G -> 1.53 G
-constexpr int kNSievePrimes = kNSmallSieve + kNLargeSieve + kNFarSieve;
+// Beyond: kNFar2Sets sets of 21 primes (257..787), one item per lane in
+// each, their state in LDS (q | s mod q << 10 | next bit << 20 | kind << 30),
+// the next bit carried from chunk to chunk by the strike loop itself; for
+// sums from 2^18 (odd sums to 2^21: 25.5 -> 21.9 ms, map reads 1.53 G ->
+// 1.12 G).
+#ifndef FFX_FAR2_SETS
+#define FFX_FAR2_SETS 4
+#endif
+constexpr int kNFar2Sets = FFX_FAR2_SETS;
+constexpr int kNSievePrimes = kNSmallSieve + kNLargeSieve + kNFarSieve + 21 * kNFar2Sets;
+constexpr int kFar2First = kNSmallSieve + kNLargeSieve + kNFarSieve;  // index of the first far-2 prime
 constexpr uint32_t kSieveBits = 4096;            // odd d per chunk (512 bytes of LDS per wave)
 constexpr uint32_t kSieveMinSum = 1u << 15;
 constexpr uint32_t kSieveMinListed = 512;  // segments shorter than this are not sieved (256..1024 measured alike, 2048 0.4 % slower)
 // square roots mod each sieve prime: 256 bytes per prime (an address from
 // the prime's index and the residue alone, no offset table to read first)
-constexpr uint32_t kSieveSqrtBytes = kNSievePrimes * 256;
-static_assert(2 * 181 - 2 > kSievePrimes[kNSievePrimes - 1], "sieve primes below the smallest probe value of a sum >= kSieveMinSum");
-static_assert(kNSievePrimes <= 51, "square roots and inverses are single bytes (primes below 255)");
+constexpr uint32_t kSqrtStride = 1024;  // entries per prime (primes below 1024)
+constexpr uint32_t kSieveSqrtBytes = kNSievePrimes * kSqrtStride * 2;
+static_assert(2 * 181 - 2 > kSievePrimes[kFar2First - 1], "sieve primes below the smallest probe value of a sum >= kSieveMinSum");
+// the far-2 sets only for sums from 2^18: their probe values exceed
+// 2 sqrt(2^18 - 1) - 2 = 1022 > every far-2 prime
+constexpr uint32_t kFar2MinSum = 1u << 18;
+static_assert(kNFar2Sets == 0 || 1022 > kSievePrimes[kNSievePrimes - 1], "far-2 primes below the probe values of a sum >= kFar2MinSum");
+static_assert(kSievePrimes[kNSievePrimes - 1] < kSqrtStride, "square roots and inverses are 16-bit, one row per prime");
 
 struct SearchArgs {
     const uint8_t *map;
@@ -161,8 +184,8 @@ struct SearchArgs {
     uint32_t bySum;           // slot of sum s is slots[(s - sumBase) / 2] instead of slot g
     uint32_t sumBase;
     uint64_t slotCap;         // bySum: slots outside [0, slotCap) are not written
-    const uint32_t *sieveK;   // per sieve prime i and k < 32: 2^k mod q | (1-4^k)^-1 << 8 | (2^(k+1))^-1 << 16 | inverse-exists << 24
-    const uint8_t *sieveSqrt; // per sieve prime, q bytes: a square root of each residue, 0xFF for non-residues
+    const uint32_t *sieveK;   // per sieve prime i and k < 32: (1-4^k)^-1 (0 if none) | (2^(k+1))^-1 << 16, mod q
+    const uint16_t *sieveSqrt;  // per sieve prime, kSqrtStride entries: a square root of each residue, 0xFFFF for non-residues
 };
 
 // What the AllButOne check reads: the map and the smallest-factor table.  It
@@ -676,12 +699,12 @@ __device__ __forceinline__ void sieve_item_for_sum(SieveItem &it, uint32_t &smal
 __device__ __forceinline__ uint2 item_root(uint32_t q, uint32_t sm, uint32_t pk, uint32_t rec, uint32_t r, int which)
 {
     uint2 it = make_uint2(0u, 0u);
-    const uint32_t iY = (rec >> 8) & 255, iX = (rec >> 16) & 255;
+    const uint32_t iY = rec & 0xFFFFu, iX = rec >> 16;  // iY = 0: 1 - 4^k = 0 mod q
     if (which == 0) {
         const uint32_t rhs = (2 + q - (sm * pk) % q) % q;  // d (1 - 4^k) = 2 - s 2^k
-        if (rec >> 24) it = make_uint2((rhs * iY) % q, 1u);
+        if (iY) it = make_uint2((rhs * iY) % q, 1u);
         else if (rhs == 0) it.y = 2;  // 4^k = 1 mod q: the value is s 2^k - 2 for every d
-    } else if (r != 0xFFu) {
+    } else if (r != 0xFFFFu) {
         it = make_uint2((((which == 1 ? sm + r : sm + q - r) % q) * iX) % q, 1u);
     }
     return it;
@@ -693,7 +716,7 @@ __device__ __forceinline__ uint2 item_root(uint32_t q, uint32_t sm, uint32_t pk,
 // table read (record, square root) is issued before any is used: one memory
 // latency per k-segment instead of four.
 __device__ __noinline__ uint3 sieve_items_for_k(uint32_t q, uint32_t sm, uint32_t small, uint32_t far,
-                                                const uint32_t *sieveK, const uint8_t *sieveSqrt, uint32_t k, int lane)
+                                                const uint32_t *sieveK, const uint16_t *sieveSqrt, uint32_t k, int lane)
 {
     const int which = lane % 3;
     const bool vS = lane < 3 * kNLargeSieve, vM = lane < 3 * kNSmallSieve, vF = lane < 3 * kNFarSieve;
@@ -706,13 +729,62 @@ __device__ __noinline__ uint3 sieve_items_for_k(uint32_t q, uint32_t sm, uint32_
     const uint32_t DF = (smF * smF + 4 * pkF * ((pkF + qF - 2) % qF)) % qF;
     const uint32_t recS = vS ? sieveK[piS * 32 + k] : 0u, recM = vM ? sieveK[piM * 32 + k] : 0u;
     const uint32_t recF = vF ? sieveK[piF * 32 + k] : 0u;
-    const uint32_t rS = vS && which ? sieveSqrt[piS * 256 + DS] : 0xFFu;
-    const uint32_t rM = vM && which ? sieveSqrt[piM * 256 + DM] : 0xFFu;
-    const uint32_t rF = vF && which ? sieveSqrt[piF * 256 + DF] : 0xFFu;
+    const uint32_t rS = vS && which ? sieveSqrt[piS * kSqrtStride + DS] : 0xFFFFu;
+    const uint32_t rM = vM && which ? sieveSqrt[piM * kSqrtStride + DM] : 0xFFFFu;
+    const uint32_t rF = vF && which ? sieveSqrt[piF * kSqrtStride + DF] : 0xFFFFu;
     const uint2 a = vS ? item_root(q, sm, pkS, recS, rS, which) : make_uint2(0u, 0u);
     const uint2 b = vM ? item_root(qM, smM, pkM, recM, rM, which) : make_uint2(0u, 0u);
     const uint2 c = vF ? item_root(qF, smF, pkF, recF, rF, which) : make_uint2(0u, 0u);
     return make_uint3(a.x | a.y << 8, b.x | b.y << 8, c.x | c.y << 8);
+}
+
+using lds_u32 = __attribute__((address_space(3))) uint32_t;
+
+// Far-2 items: per sum, q and s mod q of each lane's item in every set
+// (s = 0: none, every item marks nothing).
+__device__ __forceinline__ void far2_for_sum(lds_u32 *far2, uint32_t s, int lane)
+{
+    for (int t = 0; t < kNFar2Sets; ++t) {
+        uint32_t v = 0;
+        if (lane < 63 && s) {
+            const uint32_t q = kSievePrimes[kFar2First + 21 * t + lane / 3];
+            v = q | (s % q) << 10;
+        }
+        far2[64 * t + lane] = v;
+    }
+}
+
+// Far-2 items for 2^k: each item's root, placed as the first bit of the
+// chunk starting at odd d = dLo; every table read of all sets is issued
+// before any is used.
+__device__ __noinline__ void far2_for_k(lds_u32 *far2, const uint32_t *sieveK, const uint16_t *sieveSqrt, uint32_t k,
+                                        uint32_t dLo, int lane)
+{
+    if (lane >= 63) return;
+    const int which = lane % 3;
+    uint32_t v[kNFar2Sets], rec[kNFar2Sets], r[kNFar2Sets], pk[kNFar2Sets];
+#pragma unroll
+    for (int t = 0; t < kNFar2Sets; ++t) {
+        v[t] = far2[64 * t + lane];
+        const uint32_t q = v[t] & 1023, sm = (v[t] >> 10) & 1023, pi = kFar2First + 21 * t + lane / 3;
+        pk[t] = (1u << k) % q;
+        const uint32_t D = (sm * sm + 4 * pk[t] * ((pk[t] + q - 2) % q)) % q;
+        rec[t] = sieveK[pi * 32 + k];
+        r[t] = which ? sieveSqrt[pi * kSqrtStride + D] : 0xFFFFu;
+    }
+#pragma unroll
+    for (int t = 0; t < kNFar2Sets; ++t) {
+        const uint32_t q = v[t] & 1023, sm = (v[t] >> 10) & 1023;
+        const uint2 it = item_root(q, sm, pk[t], rec[t], r[t], which);
+        uint32_t b0 = 0;
+        if (it.y == 1) {
+            uint32_t x = it.x + q - dLo % q;
+            if (x >= q) x -= q;
+            if (x & 1) x += q;
+            b0 = x >> 1;
+        }
+        far2[64 * t + lane] = (v[t] & 0xFFFFFu) | b0 << 20 | it.y << 30;
+    }
 }
 
 // Bits 0, q, 2q, ... of a 64-bit word.
@@ -732,9 +804,8 @@ constexpr uint64_t every_qth_bit(uint32_t q)
 // large ones, so lane l strikes the first half of its own item's bits and
 // the second half of item 62-l's, and every lane carries a small and a
 // large prime.
-using lds_u32 = __attribute__((address_space(3))) uint32_t;
-__device__ __noinline__ void sieve_chunk(const SieveItem it, uint32_t small, uint32_t far, uint32_t dLo, lds_u32 *bits,
-                                        int lane)
+__device__ __noinline__ void sieve_chunk(const SieveItem it, uint32_t small, uint32_t far, lds_u32 *far2, uint32_t dLo,
+                                        lds_u32 *bits, int lane)
 {
     static_assert(3 * kNLargeSieve <= 63, "struck items fit lanes 0..62, paired l <-> 62-l");
     static_assert(kSieveBits == 64 * 64, "one 64-bit word of masked items per lane");
@@ -811,12 +882,29 @@ __device__ __noinline__ void sieve_chunk(const SieveItem it, uint32_t small, uin
         for (uint32_t b = t >> 1; b < kSieveBits; b += qF)
             __hip_atomic_fetch_or(&bits[b >> 5], 1u << (b & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
     }
+    // far-2 items: from the bit the previous chunk (or far2_for_k) left,
+    // and the bit past this chunk kept for the next one
+    if (lane < 63) {
+#pragma unroll
+        for (int t = 0; t < kNFar2Sets; ++t) {
+            const uint32_t v = far2[64 * t + lane], q2 = v & 1023, kind2 = v >> 30;
+            if (kind2 == 2) {
+                for (uint32_t i = 0; i < kSieveBits / 32; ++i)
+                    __hip_atomic_fetch_or(&bits[i], 0xFFFFFFFFu, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+            } else if (kind2 == 1) {
+                uint32_t b = (v >> 20) & 1023;
+                for (; b < kSieveBits; b += q2)
+                    __hip_atomic_fetch_or(&bits[b >> 5], 1u << (b & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+                far2[64 * t + lane] = (v & 0xC00FFFFFu) | (b - kSieveBits) << 20;
+            }
+        }
+    }
     wave_sync_lds();
 }
 
 __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const uint32_t *tab, uint32_t s, int lane,
                                                    uint2 *queue, uint2 *pq, uint4 *segs, uint8_t *segOffs,
-                                                   uint32_t *pr, uint32_t *dbits)
+                                                   uint32_t *pr, uint32_t *dbits, uint32_t *far2)
 {
     // statement 1 (RunIt :806-807)
     if (is_prime_odd(a, s - 2)) return 0;
@@ -941,7 +1029,11 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
     const bool sieveSum = s >= kSieveMinSum;
     SieveItem item;
     uint32_t small = 0, far = 1;
-    if (sieveSum) sieve_item_for_sum(item, small, far, s, lane);
+    const bool far2On = s >= kFar2MinSum;  // uniform
+    if (sieveSum) {
+        sieve_item_for_sum(item, small, far, s, lane);
+        far2_for_sum((lds_u32 *)far2, far2On ? s : 0u, lane);
+    }
     // Probes queued splits until fewer than 128 wait (all of them when
     // flush), draining the AllButOne queue in between; true once the sum
     // has two valid splits.
@@ -1009,6 +1101,7 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
                 item.rk = rk.x;
                 small = (small & 0xFFFF0000u) | rk.y;
                 far = (far & 0xFFFFu) | rk.z << 16;
+                if (far2On) far2_for_k((lds_u32 *)far2, a.sieveK, a.sieveSqrt, k, dStart, lane);
                 sj = jr;
             }
 #ifdef FFX_PROBE_COUNTERS
@@ -1016,7 +1109,7 @@ __device__ __forceinline__ uint32_t solve_sum_wave(const SearchArgs &a, const ui
             visitedSplits += segEnd - base;
 #endif
             for (uint32_t lo = dStart; lo <= dEnd; lo += 2 * kSieveBits) {
-                sieve_chunk(item, small, far, lo, (lds_u32 *)dbits, lane);
+                sieve_chunk(item, small, far, (lds_u32 *)far2, lo, (lds_u32 *)dbits, lane);
                 const uint32_t t0 = (lo % 30) >> 1;  // class of bit 0
                 const uint32_t bEnd = (dEnd - lo) >> 1;
                 const uint32_t bStar = (dStar - lo) >> 1;  // huge unless the d = o split is in the chunk
@@ -1180,6 +1273,7 @@ __global__ __launch_bounds__(kBlock, FFX_SEARCH_OCCUPANCY) void search_sums(Sear
     __shared__ uint32_t primes[kBlock][kMaxPrimes];
     __shared__ uint32_t tab[kTabWords];
     __shared__ uint32_t dsieve[kWavesPerBlock][kSieveBits / 32];
+    __shared__ uint32_t far2s[kWavesPerBlock][64 * kNFar2Sets];
     for (uint32_t i = threadIdx.x; i < kTabWords; i += kBlock) tab[i] = a.coprime[i];
     __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -1193,7 +1287,8 @@ __global__ __launch_bounds__(kBlock, FFX_SEARCH_OCCUPANCY) void search_sums(Sear
             const uint32_t slotIdx = a.nsums - 1 - idx;  // largest sums first
             const uint32_t s = sum_of_slot(a, slotIdx);
             const uint32_t res =
-                solve_sum_wave(a, tab, s, lane, queues[w], probes[w], segs[w], segOffs[w], primes[threadIdx.x], dsieve[w]);
+                solve_sum_wave(a, tab, s, lane, queues[w], probes[w], segs[w], segOffs[w], primes[threadIdx.x], dsieve[w],
+                               far2s[w]);
             if (lane == 0) {
                 if (!a.bySum) a.slots[slotIdx] = res;
                 else if (s >= a.sumBase && ((s - a.sumBase) >> 1) < a.slotCap) a.slots[(s - a.sumBase) >> 1] = res;
@@ -1316,7 +1411,7 @@ struct ffx_search_plan {
     uint32_t *dSlots;     // scratch slots for ffx_search_async
     uint64_t slotCap;
     uint32_t *dSieveK;    // d-sieve tables (per prime and k; square roots)
-    uint8_t *dSieveSqrt;
+    uint16_t *dSieveSqrt;
     int grid;
     int device;
 };
@@ -1333,7 +1428,7 @@ struct SearchTables {
     std::vector<uint16_t> spf;
     std::vector<uint32_t> coprime;
     std::vector<uint32_t> sieveK;
-    std::vector<uint8_t> sieveSqrt;
+    std::vector<uint16_t> sieveSqrt;
 };
 
 static SearchTables build_search_tables(uint32_t sumLimitMax)
@@ -1362,7 +1457,7 @@ static SearchTables build_search_tables(uint32_t sumLimitMax)
             if (g == 1 || factor) t.coprime[tab_offset(i) + r / 32] |= 1u << (r % 32);
         }
     t.sieveK.assign(kNSievePrimes * 32, 0);
-    t.sieveSqrt.assign(kSieveSqrtBytes, 0xFF);
+    t.sieveSqrt.assign(kSieveSqrtBytes / 2, 0xFFFF);
     for (int i = 0; i < kNSievePrimes; ++i) {
         const uint32_t q = kSievePrimes[i];
         auto inv = [q](uint32_t x) {
@@ -1373,11 +1468,11 @@ static SearchTables build_search_tables(uint32_t sumLimitMax)
         uint32_t pk = 1;
         for (int k = 0; k < 32; ++k) {
             const uint32_t A = (1 + q * q - pk * pk % q) % q;
-            t.sieveK[i * 32 + k] = pk | (A ? inv(A) : 0u) << 8 | inv(2 * pk % q) << 16 | (A ? 1u : 0u) << 24;
+            t.sieveK[i * 32 + k] = (A ? inv(A) : 0u) | inv(2 * pk % q) << 16;  // an inverse is never 0
             pk = 2 * pk % q;
         }
         for (uint32_t r = 0; r < q; ++r)
-            if (t.sieveSqrt[i * 256 + r * r % q] == 0xFF) t.sieveSqrt[i * 256 + r * r % q] = (uint8_t)r;
+            if (t.sieveSqrt[i * kSqrtStride + r * r % q] == 0xFFFF) t.sieveSqrt[i * kSqrtStride + r * r % q] = (uint16_t)r;
     }
     return t;
 }
@@ -1450,7 +1545,7 @@ int noalloc_setup(int device)
         return fail(FFX_EIO, "no-alloc set-up: table layout");
     FFX_HIP(hipMemcpy(base + kNaCoprimeOff, t.coprime.data(), t.coprime.size() * 4, hipMemcpyHostToDevice));
     FFX_HIP(hipMemcpy(base + kNaSieveKOff, t.sieveK.data(), t.sieveK.size() * 4, hipMemcpyHostToDevice));
-    FFX_HIP(hipMemcpy(base + kNaSqrtOff, t.sieveSqrt.data(), t.sieveSqrt.size(), hipMemcpyHostToDevice));
+    FFX_HIP(hipMemcpy(base + kNaSqrtOff, t.sieveSqrt.data(), t.sieveSqrt.size() * 2, hipMemcpyHostToDevice));
     FFX_HIP(hipMemcpy(base + kNaSpfOff, t.spf.data(), t.spf.size() * 2, hipMemcpyHostToDevice));
     int cus = 0;
     FFX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
@@ -1497,8 +1592,8 @@ int ffx_search_plan_create(ffx_search_plan **out, uint32_t sumLimitMax)
     if (e == hipSuccess) e = hipMemcpy(pl->dCoprime, t.coprime.data(), kTabWords * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMalloc(&pl->dSieveK, t.sieveK.size() * 4);
     if (e == hipSuccess) e = hipMemcpy(pl->dSieveK, t.sieveK.data(), t.sieveK.size() * 4, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMalloc(&pl->dSieveSqrt, t.sieveSqrt.size());
-    if (e == hipSuccess) e = hipMemcpy(pl->dSieveSqrt, t.sieveSqrt.data(), t.sieveSqrt.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&pl->dSieveSqrt, t.sieveSqrt.size() * 2);
+    if (e == hipSuccess) e = hipMemcpy(pl->dSieveSqrt, t.sieveSqrt.data(), t.sieveSqrt.size() * 2, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         int rc = hip_fail(e, "search plan upload");
         ffx_search_plan_destroy(pl);
@@ -1734,7 +1829,7 @@ int ffx_search_records_noalloc(int device, const uint8_t *d_map, uint64_t maxPri
     a.counter = reinterpret_cast<uint32_t *>(p + kNaCounterOff);
     a.coprime = reinterpret_cast<const uint32_t *>(p + kNaCoprimeOff);
     a.sieveK = reinterpret_cast<const uint32_t *>(p + kNaSieveKOff);
-    a.sieveSqrt = p + kNaSqrtOff;
+    a.sieveSqrt = reinterpret_cast<const uint16_t *>(p + kNaSqrtOff);
     a.grab = kGrab;
     a.eager = kEagerSplits;
     hipStream_t st = as_stream(stream);
