@@ -1503,7 +1503,7 @@ static int ensure_scratch(ffx_search_plan *pl, uint64_t nslots, bool slots)
 //
 // ffx_search_records_noalloc runs the search of ffx_search_records with no
 // allocation at call time: its work counter, tables and slots live in one
-// file-scope device array, `scratch`, of kNaBytes (6.3 MB: sums up to
+// file-scope device array, `scratch`, of kNaBytes (6.6 MB: sums up to
 // FFX_NOALLOC_MAX_SUM).  Each device holds its own copy at its own address;
 // the host takes that address once per device at ffx_init (with the device
 // current) and passes it to the kernels.  The array's name is never used as

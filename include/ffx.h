@@ -270,7 +270,7 @@ int ffx_search_records(int device, const uint8_t *d_map, uint64_t maxPrimeMapVal
                        uint64_t sumLimit, uint32_t *d_count, ffx_gpu_record *d_records);
 
 /* Largest sum ffx_search_records_noalloc accepts: its scratch array holds
- * the slots and factor table of odd sums up to here (6.3 MB per device). */
+ * the slots and factor table of odd sums up to here (6.6 MB per device). */
 #define FFX_NOALLOC_MAX_SUM 0x001FFFFFu
 
 /* No counterpart in the reference: ffx_search_records for callers that
