@@ -72,13 +72,21 @@ constexpr uint32_t group_period(int g)
 {
     return kGroupPrimes[g][0] * kGroupPrimes[g][1] * (kGroupPrimes[g][2] ? kGroupPrimes[g][2] : 1);
 }
-constexpr uint32_t pattern_words()
+// A thread fills up to kFillWords words of a segment at once, 32 words
+// apart (lane i of a half-wave on words i, i+32, i+64, ... of the
+// half-wave's block, so each read and write instruction has 32 lanes on 32
+// consecutive words: no LDS bank conflicts), with one pattern cursor per
+// group.  Each group's pattern is stored with its first kPatWrap words
+// repeated after its period, so those words never wrap inside one read.
+constexpr uint32_t kFillWords = 4;
+constexpr uint32_t kPatWrap = 32 * (kFillWords - 1);
+constexpr uint32_t pattern_base(int g)
 {
     uint32_t s = 0;
-    for (int g = 0; g < kGroups; ++g) s += group_period(g);
+    for (int i = 0; i < g; ++i) s += group_period(i) + kPatWrap;
     return s;
 }
-constexpr uint32_t kPatWords = pattern_words();  // 10238 words, 40 KiB
+constexpr uint32_t kPatWords = pattern_base(kGroups);  // 11006 words, 43 KiB
 
 // Primes with more than kCoopDiv strikes per segment strike cooperatively.
 // With the wheels, one per lane is cheaper from fewer strikes than it was
@@ -118,7 +126,7 @@ struct SieveArgs {
     uint32_t ncoop;          // primes[0, ncoop) strike wave-cooperatively (> kCoopDiv strikes per segment)
     uint32_t nwheel;         // primes[0, nwheel) of those skip their multiples of 3 (mod-6 wheel)
     uint32_t ncarry;         // lane primes carried per thread (<= the kernel's kCarry)
-    const uint32_t *patterns;  // kPatWords words, groups back to back
+    const uint32_t *patterns;  // kPatWords words: each group's period, then its first kPatWrap words again
     uint64_t segFirst, segCount;
 };
 
@@ -175,9 +183,6 @@ __device__ __forceinline__ uint32_t to_map_order(uint32_t v)
 // maps): 32 KiB with 12 re-read carries 9.45 ms (8: 9.7, none: 10.1);
 // 64 KiB with 32 held 43.4 ms on
 // 16 GiB (64 re-read: 44.9) but 214 ms on 70 GB, where 64 re-read take 202.
-#ifndef FFX_FILL_UNROLL
-#define FFX_FILL_UNROLL 4
-#endif
 // carried lane primes: the next strike's offset, bit 31 set when its
 // multiplier is 5 (mod 6) (offsets stay below 2^31: a run is shorter)
 constexpr uint32_t kFivePhase = 1u << 31;
@@ -192,22 +197,37 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kSegBy
     constexpr uint32_t kSegWords = kSegBytes / 4;
     constexpr uint32_t kSegOdds = kSegBytes * 8;  // odd values per segment
     constexpr uint32_t kCoopMax = coop_capacity(kSegOdds);  // the odd primes 67..kSegOdds/64
-    __shared__ uint32_t seg[kSegWords];
-    __shared__ uint32_t pat[kPatWords];
-    __shared__ uint32_t coopNext[kCoopMax];  // first strike of each cooperative prime, from the segment's start
+    // words per fill chunk: four in the 112 KiB kernels (two ds_read2 per
+    // pattern group), two in the 32 KiB kernel, whose 64 registers hold no
+    // more words in flight beside its carried strikes
+    constexpr uint32_t kFill = kSegBytes == 32768 ? 2 : kFillWords;
+    // the first word of this thread's chunk in a fill step of
+    // kFill * kThreads words: the wave's block, its half, its lane
+    const uint32_t fillWord0 = kFill * 64u * ((uint32_t)threadIdx.x >> 6) + kFill * 32u * (((uint32_t)threadIdx.x >> 5) & 1u) +
+                               ((uint32_t)threadIdx.x & 31u);
+    // One LDS block: the patterns first (their group offsets and the
+    // segment's own offset then fit the 16-bit offset field of the DS
+    // instructions, so neither costs an address add), then the segment
+    // (16-byte aligned for the four-word reads and writes), then the
+    // cooperative primes' next strikes.
+    constexpr uint32_t kSegAt = (kPatWords + 3) & ~3u;
+    __shared__ __attribute__((aligned(16))) uint32_t lds[kSegAt + kSegWords + kCoopMax];
+    uint32_t *const pat = lds;
+    uint32_t *const seg = lds + kSegAt;
+    uint32_t *const coopNext = lds + kSegAt + kSegWords;  // first strike of each cooperative prime, from the segment's start
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
 
     for (uint32_t i = tid; i < kPatWords; i += kThreads) pat[i] = a.patterns[i];
 
-    // Per-thread pattern cursors: byte offset 4 * ((segWord0 + tid) mod
-    // period), advanced by kThreads words per fill step and by kSegWords per
-    // segment, so no division runs inside the loops.  Kept in bytes and
-    // wrapped with an unsigned minimum, a cursor costs three instructions
-    // per word (4 GiB map 9.43 -> 9.07 ms against index cursors).
-    uint32_t period[kGroups], base[kGroups], stepFill[kGroups], stepSeg[kGroups], cur[kGroups];
-    uint32_t off = 0;
+    // Per-thread pattern cursors: a thread fills the kFill words
+    // fillWord0 + 32k (k < kFill) of each step of kFill * kThreads words;
+    // its cursor into group g is the byte offset of its first word's place
+    // in the period, advanced by the step per chunk and by kSegWords per
+    // segment, and wrapped with an unsigned minimum (no division inside the
+    // loops).
+    uint32_t period[kGroups], stepFill[kGroups], stepSeg[kGroups], cur[kGroups];
     // Each workgroup sieves a contiguous run of segments, so a cooperative
     // prime's first strike in the next segment follows from this one's
     // (kept in LDS) instead of a 64-bit division per segment.
@@ -219,18 +239,37 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kSegBy
     const uint64_t runLo = (uint64_t)blockIdx.x * perBlock;
     const uint64_t runHi = runLo + perBlock < a.segCount ? runLo + perBlock : a.segCount;
     {
-        const uint64_t w0 = (a.segFirst + runLo) * (uint64_t)kSegWords + tid;
+        const uint64_t w0 = (a.segFirst + runLo) * (uint64_t)kSegWords + fillWord0;
         const uint64_t segStride = kSegWords;
 #pragma unroll
         for (int g = 0; g < kGroups; ++g) {
             period[g] = group_period(g);
-            base[g] = off;
-            off += period[g];
             cur[g] = 4u * (uint32_t)(w0 % period[g]);
-            stepFill[g] = kThreads % period[g];
+            stepFill[g] = (kFill * kThreads) % period[g];
             stepSeg[g] = (uint32_t)(segStride % period[g]);
         }
     }
+    // The pattern words of one fill chunk (words 32k from the cursors),
+    // OR-ed group by group into dst[32k]; each cursor then moves on by one
+    // chunk step.
+    auto fill_chunk = [&](uint32_t (&c)[kGroups], uint32_t *dst) {
+        uint32_t v[kFill] = {};
+#pragma unroll
+        for (int g = 0; g < kGroups; ++g) {
+            const uint32_t *q = reinterpret_cast<const uint32_t *>(
+                reinterpret_cast<const char *>(pat + pattern_base(g)) + c[g]);
+#pragma unroll
+            for (uint32_t k = 0; k < kFill; ++k) v[k] |= q[32 * k];
+            const uint32_t n = c[g] + 4u * stepFill[g];
+            c[g] = min(n, n - 4u * period[g]);
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kFill; ++k) dst[32 * k] = v[k];
+    };
+    static_assert(kFill <= kFillWords, "the pattern wrap covers kFillWords words");
+    constexpr uint32_t kChunks = kSegWords / 4;  // write-out chunks: four words, 16 map bytes
+    static_assert(kChunks % kThreads == 0 && kSegWords % (kFill * kThreads) == 0,
+                  "every thread fills and writes the same number of chunks");
     // carried lane primes: next strike relative to the current segment's
     // start, at most the run's length (the launch keeps runs below 2^31 odd
     // values), so a prime whose p*p lies beyond the run never strikes in it
@@ -263,22 +302,13 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kSegBy
         const uint64_t j0 = segIdx * (uint64_t)kSegOdds;
         const uint64_t segByte0 = segIdx * (uint64_t)kSegBytes;
 
-        // 1. small-prime patterns
+        // 1. small-prime patterns, one chunk of four words per thread and step
         {
             uint32_t c[kGroups];
 #pragma unroll
             for (int g = 0; g < kGroups; ++g) c[g] = cur[g];
-#pragma unroll FFX_FILL_UNROLL
-            for (uint32_t w = tid; w < kSegWords; w += kThreads) {
-                uint32_t v = 0;
-#pragma unroll
-                for (int g = 0; g < kGroups; ++g) {
-                    v |= *reinterpret_cast<const uint32_t *>(reinterpret_cast<const char *>(pat + base[g]) + c[g]);
-                    const uint32_t n = c[g] + 4u * stepFill[g];
-                    c[g] = min(n, n - 4u * period[g]);
-                }
-                seg[w] = v;
-            }
+#pragma unroll 1
+            for (uint32_t w = fillWord0; w < kSegWords; w += kFill * kThreads) fill_chunk(c, seg + w);
         }
         __syncthreads();
 
@@ -465,18 +495,19 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kSegBy
         }
         __syncthreads();
 
-        // 3. write-out: 16 bytes (4 words) per thread per step
+        // 3. write-out: 16 bytes (4 words) per thread per chunk
         const uint64_t lo = a.byteLo > segByte0 ? a.byteLo : segByte0;
         const uint64_t hiSeg = segByte0 + kSegBytes;
         const uint64_t hi = a.byteHi < hiSeg ? a.byteHi : hiSeg;
-        for (uint32_t ch = tid; ch < kSegBytes / 16; ch += kThreads) {
+        for (uint32_t ch = tid; ch < kChunks; ch += kThreads) {
+            const uint4 sv = *reinterpret_cast<const uint4 *>(seg + kFillWords * ch);
             const uint64_t b0 = segByte0 + 16ull * ch;
             if (b0 + 16 <= lo || b0 >= hi) continue;
-            uint32_t out[4];
+            uint32_t out[4] = {sv.x, sv.y, sv.z, sv.w};
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 const uint32_t w = 4 * ch + k;
-                uint32_t v = seg[w];
+                uint32_t v = out[k];
                 const uint64_t jw = j0 + 32ull * w;  // odd index of bit 0
                 if (jw == 0) {
                     // value 1 is not prime; the pattern primes themselves are
@@ -551,15 +582,14 @@ static int plan_build(ffx_sieve_plan **out, std::vector<uint32_t> &oddPrimes, ui
     // word patterns: bit b of word i of group g is set when odd index 32i+b
     // is congruent to (p-1)/2 mod some prime p of the group
     std::vector<uint32_t> pat(kPatWords, 0);
-    uint32_t off = 0;
     for (int g = 0; g < kGroups; ++g) {
-        const uint32_t P = group_period(g);
+        const uint32_t P = group_period(g), off = pattern_base(g);
         for (uint64_t j = 0; j < 32ull * P; ++j)
             for (int t = 0; t < 3; ++t) {
                 const uint32_t p = kGroupPrimes[g][t];
                 if (p && j % p == (p - 1) / 2) pat[off + j / 32] |= 1u << (j % 32);
             }
-        off += P;
+        for (uint32_t i = 0; i < kPatWrap; ++i) pat[off + P + i] = pat[off + i];
     }
     ffx_sieve_plan *pl = new (std::nothrow) ffx_sieve_plan();
     if (!pl) return fail(FFX_ENOMEM, "sieve plan: host alloc");
