@@ -376,6 +376,11 @@ def main():
     if lds:
         roof_sieve["lds_strike_floor_ms"] = lds["floor_ms"]
         roof_sieve["frac_of_lds_floor"] = round(lds["floor_ms"] / sieve_avg, 3)
+        if "floor_scattered_ms" in lds:
+            # the strikes at the rate of scattered addresses, the rate the
+            # strike patterns measure at (profiles/lds_atomics.md)
+            roof_sieve["lds_strike_floor_scattered_ms"] = lds["floor_scattered_ms"]
+            roof_sieve["frac_of_lds_floor_scattered"] = round(lds["floor_scattered_ms"] / sieve_avg, 3)
 
     cpu = None if args.no_cpu else cpu_baseline(args.workload, args.cpu_timeout, args.cpu_sample)
     cpu_ref = profile_json("cpu_baseline.json").get(args.workload)
