@@ -10,7 +10,7 @@
 //
 // Kernel design (gfx950, wave64, 160 KiB LDS per CU):
 //  * One workgroup of 1024 threads owns one map segment at a time in LDS
-//    (32 KiB = 2^18 odd values below maps of 2^38 values, 112 KiB from there)
+//    (32 KiB = 2^18 odd values below maps of 2^35 values, 112 KiB from there)
 //    and sieves a contiguous run of segments.
 //  * Bits are kept "struck" = 1 in LDS, in natural order (bit b of word w is
 //    odd index 32w+b); the write-out inverts them and reorders each byte to
@@ -36,18 +36,19 @@
 namespace {
 
 // Defaults measured on MI355X (ff_sieve --autotune and direct timing):
-// maps whose sieving primes stay below 2^19 (M < 2^38) are fastest with
+// maps whose sieving primes stay below 2^17.5 (M < 2^35) are fastest with
 // 32 KiB segments and four 1024-thread workgroups per CU (two resident with
-// the 40 KiB of patterns): 4 GiB map 9.0 ms against 9.4 with 112 KiB.
-// Larger maps, where most primes strike a segment at most a few times and
-// each segment pays every prime's first-multiple computation, are faster
-// with the largest segment the LDS holds beside the patterns, one
-// workgroup resident per CU: 16 GiB map 39.5 ms with 112 KiB against 40.1
-// (96 KiB), 41.2 (80 KiB) and 41.8 (64 KiB, two per CU).  The grid holds two
-// workgroups per CU (two waves of shorter runs, 0.5 % faster than one).
+// the 43 KiB of patterns): 1 GiB map 1.64 ms against 1.70 with 112 KiB.
+// Larger maps, where most primes strike a segment only a few times and
+// each segment pays every prime's set-up, are faster with the largest
+// segment the LDS holds beside the patterns, one workgroup resident per
+// CU: 2 GiB map 3.32 ms against 3.41 with 32 KiB, 4 GiB 6.68 against 7.31,
+// 16 GiB 39.5 ms against 40.1 (96 KiB), 41.2 (80 KiB) and 41.8 (64 KiB, two
+// per CU) when the kernel had no wheels.  The grid holds two workgroups per
+// CU (one: 16 GiB 28.49 ms, two: 28.30, three: 28.33, four: 28.31).
 constexpr uint32_t kSmallMapSegBytes = 32768, kSmallMapBlocksPerCU = 4;
 constexpr uint32_t kLargeMapSegBytes = 114688, kLargeMapBlocksPerCU = 2;
-constexpr uint64_t kLargeMapFrom = 1ull << 38;
+constexpr uint64_t kLargeMapFrom = 1ull << 35;
 constexpr int kThreads = 1024;
 // Cooperative primes up to segOdds / kWheelDiv strike through a mod-6 wheel
 // (a third fewer LDS atomics; above it a lane has too few strikes per
