@@ -184,9 +184,40 @@ __device__ __forceinline__ uint32_t to_map_order(uint32_t v)
 // maps): 32 KiB with 12 re-read carries 9.45 ms (8: 9.7, none: 10.1);
 // 64 KiB with 32 held 43.4 ms on
 // 16 GiB (64 re-read: 44.9) but 214 ms on 70 GB, where 64 re-read take 202.
-// carried lane primes: the next strike's offset, bit 31 set when its
-// multiplier is 5 (mod 6) (offsets stay below 2^31: a run is shorter)
-constexpr uint32_t kFivePhase = 1u << 31;
+// Carried lane primes strike through a mod-30 wheel over the multiplier m:
+// the odd multiples with 3 | m or 5 | m are the patterns' (3 and 5 are
+// pattern primes), so only m coprime to 30 are struck -- m = 1, 7, 11, 13,
+// 17, 19, 23, 29 (mod 30), odd-index steps 3p, 2p, p, 2p, p, 2p, 3p, p from
+// one to the next (nibble i of kWheel30Steps for position i).  A carried
+// strike is its offset (bits 0..28) and the wheel position of its m (bits
+// 29..31); the launch keeps a run at most kRunOddsLimit odd values, so the
+// offsets fit.
+constexpr uint32_t kWheel30Steps = 0x13212123u;
+constexpr bool wheel30_tables_agree()
+{
+    constexpr uint32_t m[9] = {1, 7, 11, 13, 17, 19, 23, 29, 31};
+    constexpr uint64_t pos = 0x0788685483281880ull;
+    for (uint32_t i = 0; i < 8; ++i)
+        if (((kWheel30Steps >> (4 * i)) & 15u) != (m[i + 1] - m[i]) / 2) return false;
+    for (uint32_t r = 1; r < 30; r += 2) {
+        uint32_t want = 8;
+        for (uint32_t i = 0; i < 8; ++i)
+            if (m[i] == r) want = i;
+        if (((pos >> (4 * (r >> 1))) & 15u) != want) return false;
+    }
+    return true;
+}
+static_assert(wheel30_tables_agree(), "mod-30 wheel steps and positions");
+constexpr int kWheelPosShift = 29;
+constexpr uint32_t kOffsetMask = (1u << kWheelPosShift) - 1;
+constexpr uint64_t kRunOddsLimit = 1ull << kWheelPosShift;
+// the wheel position of m mod 30, or 8 when 3 | m or 5 | m (odd m)
+__device__ __forceinline__ uint32_t wheel30_pos(uint32_t r30)
+{
+    // nibble r30 / 2 (odd r30 in 1..29)
+    constexpr uint64_t kPos = 0x0788685483281880ull;
+    return (uint32_t)(kPos >> (4 * (r30 >> 1))) & 15u;
+}
 constexpr int kCarrySmall = 12;      // 32 KiB segments, primes re-read
 constexpr int kCarryHeld = 32;       // large segments, primes held
 constexpr int kCarryReread = 64;     // large segments, primes re-read
@@ -273,8 +304,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kSegBy
     static_assert(kChunks % kThreads == 0 && kSegWords % (kFill * kThreads) == 0,
                   "every thread fills and writes the same number of chunks");
     // carried lane primes: next strike relative to the current segment's
-    // start, at most the run's length (the launch keeps runs below 2^31 odd
-    // values), so a prime whose p*p lies beyond the run never strikes in it
+    // start, at most the run's length (at most kRunOddsLimit odd values), so
+    // a prime whose p*p lies beyond the run never strikes in it
     uint32_t cn[kCarry], cp[kHoldPrimes ? kCarry : 1];
     if (runLo < runHi) {
         const uint64_t j0 = (a.segFirst + runLo) * (uint64_t)kSegOdds;
@@ -287,12 +318,19 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kSegBy
             if ((uint32_t)k < a.ncarry && pi < a.nprimes) {
                 const uint32_t p = a.primes[pi];
                 uint64_t f = first_strike(j0, p, a.invp[pi]);
-                // mod-6 wheel: its multiple m (exact, as for the cooperative
-                // primes); a multiple of 3 moves on to m + 2 = 5 (mod 6)
+                // mod-30 wheel: its multiple m (exact, as for the cooperative
+                // primes), moved on by 2 (p odd indices) at most twice to
+                // the next m prime to 30
                 const uint64_t m = (uint64_t)rint((2.0 * (double)(j0 + f) + 1.0) * a.invp[pi]);
-                const uint32_t r6 = (uint32_t)(m % 6);
-                if (r6 == 3) f += p;
-                cn[k] = (uint32_t)(f < runOdds ? f : runOdds) | (r6 != 1 ? kFivePhase : 0u);
+                uint32_t r30 = (uint32_t)(m % 30), pos = wheel30_pos(r30);
+#pragma unroll
+                for (int t = 0; t < 2; ++t)
+                    if (pos == 8) {
+                        f += p;
+                        r30 = r30 + 2 == 31 ? 1u : r30 + 2;
+                        pos = wheel30_pos(r30);
+                    }
+                cn[k] = (uint32_t)(f < runOdds ? f : runOdds) | pos << kWheelPosShift;
                 if (kHoldPrimes) cp[kHoldPrimes ? k : 0] = p;
             }
         }
@@ -459,16 +497,17 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kSegBy
                     pnext = cpr[(k + 1) * kThreads];
                 }
                 if ((uint32_t)k < a.ncarry && lane0 + (uint32_t)k * kThreads < a.nprimes) {
-                    // mod-6 wheel over the multiplier: from m = 1 (mod 6) the
-                    // next m = 5 is 2p odd indices on, from m = 5 the next
-                    // m = 1 is p on; the odd multiples of 3 are the pattern's
-                    uint32_t j = cn[k] & ~kFivePhase, st = (cn[k] & kFivePhase) ? p : 2 * p;
-                    const uint32_t p3 = 3 * p;
-                    for (; j < limit; st = p3 - st) {
+                    // mod-30 wheel: pos4 is four times the wheel position of
+                    // the next strike (the bit-field extract takes it mod 32)
+                    uint32_t j = cn[k] & kOffsetMask, pos4 = (cn[k] >> kWheelPosShift) << 2;
+                    uint32_t st = __umul24(p, __builtin_amdgcn_ubfe(kWheel30Steps, pos4, 4));
+                    while (j < limit) {
                         atomicOr(&seg[j >> 5], 1u << (j & 31));
                         j += st;
+                        pos4 += 4;
+                        st = __umul24(p, __builtin_amdgcn_ubfe(kWheel30Steps, pos4, 4));
                     }
-                    cn[k] = (j - kSegOdds) | (st == p ? kFivePhase : 0u);
+                    cn[k] = (j - kSegOdds) | ((pos4 >> 2) & 7u) << kWheelPosShift;
                 }
             }
             // primes beyond the carried ones: first strike from j0 in
@@ -669,12 +708,14 @@ static int launch(const ffx_sieve_plan *pl, uint8_t *dst, uint64_t dstBase, uint
     if (a.segFirst * segOdds >= (1ull << 52)) return fail(FFX_ERANGE, "sieve: map offset beyond 2^53 values");
     const uint64_t maxGrid = (uint64_t)pl->cus * pl->blocksPerCU;
     uint64_t grid = a.segCount < maxGrid ? a.segCount : maxGrid;
-    // each workgroup's run of segments stays below 2^31 odd values (its
-    // carried strike offsets are 32-bit); when that takes more workgroups
-    // than the grid, a whole multiple of the CU count keeps the last wave
-    // from running a few workgroups alone (70 GB map, one 112 KiB workgroup
-    // per CU: 261 runs took 357 ms, 512 take 188)
-    const uint64_t minGrid = (a.segCount * segOdds + (1ull << 31) - 1) >> 31;
+    // each workgroup's run of segments stays within kRunOddsLimit odd values
+    // (its carried strike offsets have 29 bits); when that takes more
+    // workgroups than the grid, a whole multiple of the CU count keeps the
+    // last wave from running a few workgroups alone (70 GB map, one 112 KiB
+    // workgroup per CU, runs of at most 2^31 values: 261 runs took 357 ms,
+    // 512 take 188)
+    const uint64_t maxRun = kRunOddsLimit / segOdds;  // segments per run
+    const uint64_t minGrid = (a.segCount + maxRun - 1) / maxRun;
     if (grid < minGrid) grid = std::min<uint64_t>((minGrid + pl->cus - 1) / pl->cus * pl->cus, a.segCount);
     const bool hold = pl->holdPrimes == 1 || (pl->holdPrimes == 0 && a.nprimes - a.ncoop <= kHeldUpTo);
     if (segBytes == 32768) {
