@@ -249,7 +249,11 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kSegBy
     uint32_t *const coopNext = lds + kSegAt + kSegWords;  // first strike of each cooperative prime, from the segment's start
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    const int wave = tid >> 6;
+    // The kernel that re-reads its lane primes takes the wave index as
+    // wave-uniform, so the cooperative loop's index lives in a scalar
+    // register instead of a spilled vector one (70 GB map 144.9 -> 137.3 ms);
+    // the other kernels measured 1 % slower with it.
+    const int wave = (kSegBytes != 32768 && !kHoldPrimes) ? __builtin_amdgcn_readfirstlane(tid >> 6) : tid >> 6;
 
     for (uint32_t i = tid; i < kPatWords; i += kThreads) pat[i] = a.patterns[i];
 
