@@ -59,6 +59,13 @@ constexpr int kThreads = 1024;
 #define FFX_SIEVE_WHEEL_DIV 384
 #endif
 constexpr uint32_t kWheelDiv = FFX_SIEVE_WHEEL_DIV;
+// Those up to segOdds / kWheel30Div also skip their multiples of 5 (a fifth
+// fewer strikes again); above it a lane has too few strikes per segment to
+// pay for the wheel's set-up and partial last step.
+#ifndef FFX_SIEVE_WHEEL30_DIV
+#define FFX_SIEVE_WHEEL30_DIV 1536
+#endif
+constexpr uint32_t kWheel30Div = FFX_SIEVE_WHEEL30_DIV;
 constexpr int kWaves = kThreads / 64;
 
 constexpr int kGroups = 8;
@@ -126,6 +133,7 @@ struct SieveArgs {
     uint32_t nprimes;
     uint32_t ncoop;          // primes[0, ncoop) strike wave-cooperatively (> kCoopDiv strikes per segment)
     uint32_t nwheel;         // primes[0, nwheel) of those skip their multiples of 3 (mod-6 wheel)
+    uint32_t nwheel30;       // primes[0, nwheel30) of those also skip their multiples of 5 (mod-30 wheel)
     uint32_t ncarry;         // lane primes carried per thread (<= the kernel's kCarry)
     const uint32_t *patterns;  // kPatWords words: each group's period, then its first kPatWrap words again
     uint64_t segFirst, segCount;
@@ -366,7 +374,109 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kSegBy
             const uint32_t half = (uint32_t)lane >> 5, hl = (uint32_t)lane & 31;
             for (uint32_t pb = 2 * (uint32_t)wave; pb < a.ncoop; pb += 2 * kWaves) {
                 const uint32_t pi = pb + half;
-                if (pi < a.ncoop && pi < a.nwheel) {
+                if (pi < a.ncoop && pi < a.nwheel30) {
+                    // Mod-30 wheel: the odd multiples m*p with 3 | m or
+                    // 5 | m are already struck (3 and 5 are pattern
+                    // primes).  Two progressions of odd-index stride 3p
+                    // hold m = 1 and m = 5 (mod 6); lane l of the half
+                    // takes elements l + 32t of each, a stride of 96p odd
+                    // values = 3p whole words, so its bit stays put.  Along
+                    // a lane m grows by 192 = 2 (mod 5) per element, so one
+                    // element in five is a multiple of 5 and is stepped
+                    // over: each step of the lane's loop strikes 4 of 5
+                    // elements at offsets fixed for the lane.  coopNext
+                    // keeps the first element at or past the segment's
+                    // start as offset << 4 | (m mod 5) << 1 | (m = 5 mod 6).
+                    // Larger wheel primes take the mod-6 block below: its
+                    // set-up is shorter, and the kernel is issue-bound.
+                    const uint32_t p = a.primes[pi];
+                    uint32_t first, five, first5;
+                    if (si == runLo) {
+                        uint64_t f = first_strike(j0, p, a.invp[pi]);  // < (p*p-1)/2 < 2^32
+                        // its multiple m (exact: 2(j0+f)+1 < 2^53 is m*p)
+                        const uint64_t m = (uint64_t)rint((2.0 * (double)(j0 + f) + 1.0) * a.invp[pi]);
+                        uint32_t r30 = (uint32_t)(m % 30);
+                        if (r30 % 6 == 3) {  // m + 2 = 5 (mod 6): one multiple on
+                            f += p;
+                            r30 += 2;
+                        }
+                        first = (uint32_t)f;
+                        five = r30 % 6 == 5;
+                        first5 = r30 % 5;
+                    } else {
+                        const uint32_t v = coopNext[pi];
+                        first = v >> 4;
+                        first5 = (v >> 1) & 7u;
+                        five = v & 1u;
+                    }
+                    const uint32_t p3 = 3 * p;  // < 2^16
+                    const uint32_t stepBytes = 4u * p3, endBytes = ((limit + 31) >> 5) << 2;
+                    char *segBytes = reinterpret_cast<char *>(seg);
+#pragma unroll
+                    for (int r = 0; r < 2; ++r) {
+                        // the progression of first (r = 0), then the other
+                        // residue: from m = 1 the next m = 5 is m + 4
+                        // (index + 2p), from m = 5 the next m = 1 is m + 2
+                        // (index + p)
+                        const uint32_t fr = r == 0 ? first : first + (five ? p : 2 * p);
+                        const uint32_t fiveR = r == 0 ? five : 1u - five;
+                        const uint32_t fr5 = r == 0 ? first5 : first5 + (five ? 2u : 4u);
+                        // m mod 5 of the lane's element 0 (m + 6l), then of
+                        // element t: c5 + 2t, a multiple of 5 at t = 2 c5
+                        // (mod 5)
+                        const uint32_t c5 = (fr5 + hl) % 5u;
+                        const uint32_t jr = fr + __umul24(hl, p3);
+                        uint32_t je = jr, t = 0;
+                        if (jr < limit) {
+                            const uint32_t mask = 1u << (jr & 31);
+                            const uint32_t skip = (2u * c5) % 5u;
+                            uint32_t ad = (jr >> 5) << 2;
+                            const uint32_t o0 = skip == 0 ? stepBytes : 0u, o1 = o0 + (skip == 1 ? 2u : 1u) * stepBytes,
+                                           o2 = o1 + (skip == 2 ? 2u : 1u) * stepBytes,
+                                           o3 = o2 + (skip == 3 ? 2u : 1u) * stepBytes;
+                            // a step covers five elements and runs only when
+                            // the fifth is below the end too, even when it is
+                            // the one stepped over (skip == 4, o3 = 3 steps),
+                            // so the lane stops on its first element past it
+                            const uint32_t endGroup =
+                                skip == 4 ? (endBytes > stepBytes ? endBytes - stepBytes : 0u) : endBytes;
+                            uint32_t a0 = ad + o0, a1 = ad + o1, a2 = ad + o2, a3 = ad + o3;
+                            for (; a3 < endGroup; a0 += 5 * stepBytes, a1 += 5 * stepBytes, a2 += 5 * stepBytes,
+                                                  a3 += 5 * stepBytes) {
+                                atomicOr(reinterpret_cast<uint32_t *>(segBytes + a0), mask);
+                                atomicOr(reinterpret_cast<uint32_t *>(segBytes + a1), mask);
+                                atomicOr(reinterpret_cast<uint32_t *>(segBytes + a2), mask);
+                                atomicOr(reinterpret_cast<uint32_t *>(segBytes + a3), mask);
+                                // four addresses, each advanced once per step
+                                // (left to itself the compiler recomputes
+                                // them from a common offset, six more VALU)
+                                asm("" : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3));
+                            }
+                            // the last, partial step: at most four elements
+                            // below the end, struck in offset order (a3 is
+                            // below it only when skip == 4), and t of them
+                            ad = a0 - o0;
+                            if (a0 < endBytes) atomicOr(reinterpret_cast<uint32_t *>(segBytes + a0), mask);
+                            if (a1 < endBytes) atomicOr(reinterpret_cast<uint32_t *>(segBytes + a1), mask);
+                            if (a2 < endBytes) atomicOr(reinterpret_cast<uint32_t *>(segBytes + a2), mask);
+                            if (a3 < endBytes) atomicOr(reinterpret_cast<uint32_t *>(segBytes + a3), mask);
+                            t = (ad < endBytes) + (ad + stepBytes < endBytes) + (ad + 2 * stepBytes < endBytes) +
+                                (ad + 3 * stepBytes < endBytes);
+                            je = ((ad + t * stepBytes) << 3) | (jr & 31);
+                        }
+                        // the 32 lanes end on 32 consecutive elements at or
+                        // past kSegOdds; the one ending below kSegOdds + 3p
+                        // (lane 0 when the progression did not strike) holds
+                        // its first in the next segment, and the smaller of
+                        // the two progressions' firsts is kept
+                        const uint32_t d = je - kSegOdds;
+                        if (limit == kSegOdds && (d < p3 || (hl == 0 && fr >= kSegOdds))) {
+                            const uint32_t v = d << 4 | ((c5 + 2u * t) % 5u) << 1 | fiveR;
+                            if (r == 0) coopNext[pi] = v;
+                            else atomicMin(&coopNext[pi], v);
+                        }
+                    }
+                } else if (pi < a.ncoop && pi < a.nwheel) {
                     // Mod-6 wheel: the odd multiples m*p with 3 | m are
                     // already struck (3 is a pattern prime), so only
                     // m = 1 and m = 5 (mod 6) are struck, as two progressions
@@ -706,6 +816,9 @@ static int launch(const ffx_sieve_plan *pl, uint8_t *dst, uint64_t dstBase, uint
     // the mod-6 wheel for the cooperative primes up to segOdds / kWheelDiv
     a.nwheel = (uint32_t)(std::lower_bound(pl->hostPrimes.begin(), pl->hostPrimes.end(), (uint32_t)(segOdds / kWheelDiv)) -
                           pl->hostPrimes.begin());
+    a.nwheel30 = std::min(a.nwheel, (uint32_t)(std::lower_bound(pl->hostPrimes.begin(), pl->hostPrimes.end(),
+                                                                 (uint32_t)(segOdds / kWheel30Div)) -
+                                               pl->hostPrimes.begin()));
     a.patterns = pl->dPatterns;
     a.segFirst = byteLo / segBytes;
     a.segCount = (byteHi + segBytes - 1) / segBytes - a.segFirst;
