@@ -229,7 +229,10 @@ __device__ __forceinline__ uint32_t wheel30_pos(uint32_t r30)
 constexpr int kCarrySmall = 12;      // 32 KiB segments, primes re-read
 constexpr int kCarryHeld = 32;       // large segments, primes held
 constexpr int kCarryReread = 64;     // large segments, primes re-read
-constexpr uint32_t kHeldUpTo = 48 * 1024;  // lane primes up to which the large kernel holds them
+// The large kernels hold their carried primes by default: 32 held against
+// 64 re-read, 4 GiB map 5.85 / 6.66 ms, 16 GiB 25.9 / 27.9, 70 GB 133.8 /
+// 136.8, 256 GiB 764.7 / 770.0 (more primes carried does not make up for the
+// registers the re-read kernel spends on them).
 
 template <uint32_t kSegBytes, int kCarry, bool kHoldPrimes>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kSegBytes == 32768 ? 8 : 4))) void sieve_segments(SieveArgs a)
@@ -731,7 +734,7 @@ struct ffx_sieve_plan {
     std::vector<uint32_t> hostPrimes;  // the struck primes (> 61)
     uint32_t segBytes, blocksPerCU, cus;
     uint32_t laneCarry;   // at most this many carried lane primes per thread (FFX_SIEVE_LANE_CARRY)
-    uint32_t holdPrimes;  // large segments: 0 by the lane-prime count, 1 held, 2 re-read
+    uint32_t holdPrimes;  // large segments: 0 (held) or 1 held, 2 re-read
     uint32_t *dPrimes;
     double *dInv;
     uint32_t *dPatterns;
@@ -834,7 +837,7 @@ static int launch(const ffx_sieve_plan *pl, uint8_t *dst, uint64_t dstBase, uint
     const uint64_t maxRun = kRunOddsLimit / segOdds;  // segments per run
     const uint64_t minGrid = (a.segCount + maxRun - 1) / maxRun;
     if (grid < minGrid) grid = std::min<uint64_t>((minGrid + pl->cus - 1) / pl->cus * pl->cus, a.segCount);
-    const bool hold = pl->holdPrimes == 1 || (pl->holdPrimes == 0 && a.nprimes - a.ncoop <= kHeldUpTo);
+    const bool hold = pl->holdPrimes != 2;
     if (segBytes == 32768) {
         a.ncarry = std::min<uint32_t>(pl->laneCarry, kCarrySmall);
         hipLaunchKernelGGL((sieve_segments<32768, kCarrySmall, false>), dim3((uint32_t)grid), dim3(kThreads), 0,

@@ -144,7 +144,7 @@ int ffx_sieve_plan_destroy(ffx_sieve_plan *plan);
                                       strike across segments (0..64, default 64; the kernel holds
                                       at most 12 with 32 KiB segments, 32 or 64 with 112 KiB) */
 #define FFX_SIEVE_HOLD_PRIMES 4    /* 112 KiB segments: carried primes held in registers (1) or
-                                      re-read each segment (2); 0 (default) picks by their count */
+                                      re-read each segment (2); 0 (default) holds them */
 int ffx_sieve_plan_set(ffx_sieve_plan *plan, int param, uint32_t value);
 int ffx_sieve_plan_get(const ffx_sieve_plan *plan, int param, uint32_t *value);
 
