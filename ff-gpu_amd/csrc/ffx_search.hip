@@ -238,6 +238,25 @@ __device__ __forceinline__ double mulmod_f64(double a, double b, double n, doubl
 // is below 2^50 for sums up to FFX_MAX_SUM (2^26 - 1): the probe values
 // 2^k + d*o - 2 <= s^2/16 + 4 and d + o*2^k - 2 < s, the AllButOne terms
 // f + P/f <= P/2 + 2 with P <= s^2/4, and s - 2, s/3 and the terms of s.
+//
+// For 2^20 <= n < 2^47 (every probe value of sums below 2^25) residues are
+// kept balanced, |y| < 0.53 n, and a product is reduced by the nearest
+// quotient: h = a*b rounded, l = a*b - h exactly (fma), q = rint(h/n) is
+// within 0.52 of a*b/n (|a*b| < 0.29 n^2 and n < 2^47 put the reciprocal's
+// error below 0.01), so h - q*n (one exact fma: an integer below 2^48) plus
+// l (|l| <= |h| 2^-52 < 0.01 n) is again below 0.53 n -- six operations, no
+// corrections.  The exponent is taken left to right, so the other factor
+// of each step is the base itself: b*y < 17 * 0.53 n < 2^51 is exact and
+// its reduction is a rint and one fma.  y = 1 and y = -1 are then the only
+// residues of 1 and n - 1, so every test reads exactly as in the general
+// path below.
+__device__ __forceinline__ double mulmod_bal(double a, double b, double n, double ninv)
+{
+    const double h = a * b;
+    const double l = fma(a, b, -h);
+    return fma(-rint(h * ninv), n, h) + l;
+}
+
 __device__ __noinline__ bool mr_is_prime(uint64_t n)
 {
     FFX_COUNT(8, 1);
@@ -246,6 +265,32 @@ __device__ __noinline__ bool mr_is_prime(uint64_t n)
     const uint64_t d = (n - 1) >> r;
     const int nb = n < 3215031751ull ? 4 : n < 2152302898747ull ? 5 : n < 3474749660383ull ? 6 : 7;
     const double bases[7] = {2, 3, 5, 7, 11, 13, 17};
+    if (n >= (1ull << 20) && n < (1ull << 47)) {
+        const int top = 63 - __builtin_clzll(d);  // d is odd: bit 0 set
+        for (int bi = 0; bi < nb; ++bi) {
+            const double b = bases[bi];
+            double y = b;
+            for (int i = top - 1; i >= 0; --i) {
+                y = mulmod_bal(y, y, nd, ninv);
+                if ((d >> i) & 1) {
+                    const double h = y * b;  // exact
+                    y = fma(-rint(h * ninv), nd, h);
+                }
+            }
+            if (y == 1.0 || y == -1.0) continue;
+            bool pass = false;
+            for (uint32_t j = 1; j < r; ++j) {
+                y = mulmod_bal(y, y, nd, ninv);
+                if (y == -1.0) {
+                    pass = true;
+                    break;
+                }
+                if (y == 1.0) return false;
+            }
+            if (!pass) return false;
+        }
+        return true;
+    }
     for (int bi = 0; bi < nb; ++bi) {
         double b = bases[bi], y = 1.0;
         for (uint64_t e = d; e; e >>= 1) {
