@@ -66,7 +66,6 @@ constexpr uint32_t kWheelDiv = FFX_SIEVE_WHEEL_DIV;
 #define FFX_SIEVE_WHEEL30_DIV 1536
 #endif
 constexpr uint32_t kWheel30Div = FFX_SIEVE_WHEEL30_DIV;
-constexpr int kWaves = kThreads / 64;
 
 constexpr int kGroups = 8;
 // Small-prime groups whose word patterns replace strikes.  Each period (the
@@ -255,16 +254,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kSegBy
     // cooperative primes' next strikes.
     constexpr uint32_t kSegAt = (kPatWords + 3) & ~3u;
     __shared__ __attribute__((aligned(16))) uint32_t lds[kSegAt + kSegWords + kCoopMax];
+    __shared__ uint32_t coopTaken;  // cooperative primes taken in this segment
     uint32_t *const pat = lds;
     uint32_t *const seg = lds + kSegAt;
     uint32_t *const coopNext = lds + kSegAt + kSegWords;  // first strike of each cooperative prime, from the segment's start
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    // The kernel that re-reads its lane primes takes the wave index as
-    // wave-uniform, so the cooperative loop's index lives in a scalar
-    // register instead of a spilled vector one (70 GB map 144.9 -> 137.3 ms);
-    // the other kernels measured 1 % slower with it.
-    const int wave = (kSegBytes != 32768 && !kHoldPrimes) ? __builtin_amdgcn_readfirstlane(tid >> 6) : tid >> 6;
 
     for (uint32_t i = tid; i < kPatWords; i += kThreads) pat[i] = a.patterns[i];
 
@@ -357,6 +352,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kSegBy
         const uint64_t j0 = segIdx * (uint64_t)kSegOdds;
         const uint64_t segByte0 = segIdx * (uint64_t)kSegBytes;
 
+        if (tid == 0) coopTaken = 0;
         // 1. small-prime patterns, one chunk of four words per thread and step
         {
             uint32_t c[kGroups];
@@ -371,11 +367,85 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kSegBy
         const uint64_t limit64 = a.oddLimit > j0 ? a.oddLimit - j0 : 0;
         const uint32_t limit = limit64 < kSegOdds ? (uint32_t)limit64 : kSegOdds;
         if (limit) {
-            // two primes per wavefront, one per half: the same half owns
-            // prime pi in every segment of the run, so coopNext[pi] is
-            // written and read by one half-wave only
+            // Each wave strikes its own lane primes first, then takes the
+            // cooperative primes two at a time (one per half) from a counter
+            // until they run out, so the waves that finish their lane
+            // primes first strike more of them and the workgroup reaches
+            // the barrier together (dealt out two per wave in turn, the
+            // waves' shares differed: 16 GiB map 26.0 -> 24.5 ms, 4 GiB
+            // 5.84 -> 5.67).  A prime is struck by one half-wave per
+            // segment, whichever it is: coopNext[pi] carries its next strike
+            // to the next segment, past the barriers between.
             const uint32_t half = (uint32_t)lane >> 5, hl = (uint32_t)lane & 31;
-            for (uint32_t pb = 2 * (uint32_t)wave; pb < a.ncoop; pb += 2 * kWaves) {
+            // Carried lane primes.  After a prime's last strike j >= limit;
+            // with limit == kSegOdds, j - kSegOdds is the next segment's first
+            // strike (a segment cut short by oddLimit is the last one the run
+            // strikes).  Re-read primes: the empty asm keeps the compiler
+            // from hoisting their loads out of the segment loop (holding them
+            // after all); the pointer is then cast back to the global address
+            // space, since a flat load also counts on the LDS counter and its
+            // wait inside the strike loop would wait out every ds_or as well
+            // (4 GiB map 10.2 -> 9.9 ms, 70 GB map 241 -> 215 ms together with
+            // the one-ahead prefetch).
+            const uint32_t *cprGeneric = a.primes + a.ncoop + tid;
+            if (!kHoldPrimes) asm volatile("" : "+v"(cprGeneric));
+            const __attribute__((address_space(1))) uint32_t *cpr =
+                (const __attribute__((address_space(1))) uint32_t *)cprGeneric;
+            const uint32_t lane0 = a.ncoop + (uint32_t)tid;
+            uint32_t pnext = 0;
+            if (!kHoldPrimes && 0 < a.ncarry && lane0 < a.nprimes) pnext = cpr[0];
+#pragma unroll
+            for (int k = 0; k < kCarry; ++k) {
+                // re-read primes: prime k+1 is loaded while prime k strikes
+                uint32_t p = pnext;
+                if (kHoldPrimes) {
+                    p = cp[kHoldPrimes ? k : 0];
+                } else if (k + 1 < kCarry && (uint32_t)(k + 1) < a.ncarry &&
+                           lane0 + (uint32_t)(k + 1) * kThreads < a.nprimes) {
+                    pnext = cpr[(k + 1) * kThreads];
+                }
+                if ((uint32_t)k < a.ncarry && lane0 + (uint32_t)k * kThreads < a.nprimes) {
+                    // mod-30 wheel: pos4 is four times the wheel position of
+                    // the next strike (the bit-field extract takes it mod 32)
+                    uint32_t j = cn[k] & kOffsetMask, pos4 = (cn[k] >> kWheelPosShift) << 2;
+                    uint32_t st = __umul24(p, __builtin_amdgcn_ubfe(kWheel30Steps, pos4, 4));
+                    while (j < limit) {
+                        atomicOr(&seg[j >> 5], 1u << (j & 31));
+                        j += st;
+                        pos4 += 4;
+                        st = __umul24(p, __builtin_amdgcn_ubfe(kWheel30Steps, pos4, 4));
+                    }
+                    cn[k] = (j - kSegOdds) | ((pos4 >> 2) & 7u) << kWheelPosShift;
+                }
+            }
+            // primes beyond the carried ones: first strike from j0 in
+            // doubles, the next prime loaded while this one strikes
+            const double j0d = (double)j0;
+            uint32_t pi = a.ncoop + a.ncarry * kThreads + tid, pn = 0;
+            // opaque per segment, so the compiler forms the prime and
+            // reciprocal addresses here instead of holding them (spilled, in
+            // the 64-register kernel) through the whole run
+            asm volatile("" : "+v"(pi));
+            double ivn = 0.0;
+            if (pi < a.nprimes) {
+                pn = a.primes[pi];
+                ivn = a.invp[pi];
+            }
+            for (; pi < a.nprimes; pi += kThreads) {
+                const uint32_t p = pn;
+                const double iv = ivn;
+                if (pi + kThreads < a.nprimes) {
+                    pn = a.primes[pi + kThreads];
+                    ivn = a.invp[pi + kThreads];
+                }
+                for (uint32_t j = first_strike_below(j0d, p, iv, limit); j < limit; j += p)
+                    atomicOr(&seg[j >> 5], 1u << (j & 31));
+            }
+            for (;;) {
+                uint32_t pb = 0;
+                if (lane == 0) pb = atomicAdd(&coopTaken, 2u);
+                pb = __builtin_amdgcn_readfirstlane(pb);
+                if (pb >= a.ncoop) break;
                 const uint32_t pi = pb + half;
                 if (pi < a.ncoop && pi < a.nwheel30) {
                     // Mod-30 wheel: the odd multiples m*p with 3 | m or
@@ -585,70 +655,6 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(kSegBy
                     const uint32_t d = je - kSegOdds;
                     if (limit == kSegOdds && (d < p || (hl == 0 && first >= kSegOdds))) coopNext[pi] = d;
                 }
-            }
-            // Carried lane primes.  After a prime's last strike j >= limit;
-            // with limit == kSegOdds, j - kSegOdds is the next segment's first
-            // strike (a segment cut short by oddLimit is the last one the run
-            // strikes).  Re-read primes: the empty asm keeps the compiler
-            // from hoisting their loads out of the segment loop (holding them
-            // after all); the pointer is then cast back to the global address
-            // space, since a flat load also counts on the LDS counter and its
-            // wait inside the strike loop would wait out every ds_or as well
-            // (4 GiB map 10.2 -> 9.9 ms, 70 GB map 241 -> 215 ms together with
-            // the one-ahead prefetch).
-            const uint32_t *cprGeneric = a.primes + a.ncoop + tid;
-            if (!kHoldPrimes) asm volatile("" : "+v"(cprGeneric));
-            const __attribute__((address_space(1))) uint32_t *cpr =
-                (const __attribute__((address_space(1))) uint32_t *)cprGeneric;
-            const uint32_t lane0 = a.ncoop + (uint32_t)tid;
-            uint32_t pnext = 0;
-            if (!kHoldPrimes && 0 < a.ncarry && lane0 < a.nprimes) pnext = cpr[0];
-#pragma unroll
-            for (int k = 0; k < kCarry; ++k) {
-                // re-read primes: prime k+1 is loaded while prime k strikes
-                uint32_t p = pnext;
-                if (kHoldPrimes) {
-                    p = cp[kHoldPrimes ? k : 0];
-                } else if (k + 1 < kCarry && (uint32_t)(k + 1) < a.ncarry &&
-                           lane0 + (uint32_t)(k + 1) * kThreads < a.nprimes) {
-                    pnext = cpr[(k + 1) * kThreads];
-                }
-                if ((uint32_t)k < a.ncarry && lane0 + (uint32_t)k * kThreads < a.nprimes) {
-                    // mod-30 wheel: pos4 is four times the wheel position of
-                    // the next strike (the bit-field extract takes it mod 32)
-                    uint32_t j = cn[k] & kOffsetMask, pos4 = (cn[k] >> kWheelPosShift) << 2;
-                    uint32_t st = __umul24(p, __builtin_amdgcn_ubfe(kWheel30Steps, pos4, 4));
-                    while (j < limit) {
-                        atomicOr(&seg[j >> 5], 1u << (j & 31));
-                        j += st;
-                        pos4 += 4;
-                        st = __umul24(p, __builtin_amdgcn_ubfe(kWheel30Steps, pos4, 4));
-                    }
-                    cn[k] = (j - kSegOdds) | ((pos4 >> 2) & 7u) << kWheelPosShift;
-                }
-            }
-            // primes beyond the carried ones: first strike from j0 in
-            // doubles, the next prime loaded while this one strikes
-            const double j0d = (double)j0;
-            uint32_t pi = a.ncoop + a.ncarry * kThreads + tid, pn = 0;
-            // opaque per segment, so the compiler forms the prime and
-            // reciprocal addresses here instead of holding them (spilled, in
-            // the 64-register kernel) through the whole run
-            asm volatile("" : "+v"(pi));
-            double ivn = 0.0;
-            if (pi < a.nprimes) {
-                pn = a.primes[pi];
-                ivn = a.invp[pi];
-            }
-            for (; pi < a.nprimes; pi += kThreads) {
-                const uint32_t p = pn;
-                const double iv = ivn;
-                if (pi + kThreads < a.nprimes) {
-                    pn = a.primes[pi + kThreads];
-                    ivn = a.invp[pi + kThreads];
-                }
-                for (uint32_t j = first_strike_below(j0d, p, iv, limit); j < limit; j += p)
-                    atomicOr(&seg[j >> 5], 1u << (j & 31));
             }
         }
         __syncthreads();
