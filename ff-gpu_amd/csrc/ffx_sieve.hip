@@ -95,15 +95,21 @@ constexpr uint32_t pattern_base(int g)
 }
 constexpr uint32_t kPatWords = pattern_base(kGroups);  // 11006 words, 43 KiB
 
-// Primes with more than kCoopDiv strikes per segment strike cooperatively.
-// With the wheels, one per lane is cheaper from fewer strikes than it was
-// without them: 16 GiB map 34.1 / 32.3 / 31.5 / 30.9 / 31.2 / 31.8 / 33.2 ms
-// with 64 / 96 / 128 / 192 / 256 / 384 / 512 (70 GB map 170 -> 153 ms, 4 GiB
-// 8.1 -> 7.7 ms at 192).
-constexpr uint32_t kCoopDiv = 192;
+// Primes below segOdds / coop_div(segOdds) (more than coop_div strikes per
+// segment) strike cooperatively, the rest one per lane through the mod-30
+// wheel.  Since the cooperative primes are
+// taken from a counter after each wave's lane primes, the boundary sits
+// higher: 112 KiB segments, divisor 192 / 256 / 320 / 384 / 512 / 640 /
+// 768 / 896 / 1024: 16 GiB map 24.6 / 23.5 / 22.9 / 22.4 / 21.9 / 21.4 /
+// 21.4 / 22.9 / 24.5 ms (4 GiB 5.69 ... 4.88 at 768 ... 5.72 at 1024, 70
+// GB 128 ... 113 ... 115); 32 KiB segments, whose threads carry only 12
+// lane primes: 1 GiB map 1.54 / 1.49 / 1.48 / 1.50 / 1.66 ms at 192 /
+// 256 / 320 / 384 / 512.  (Before the counter, 192 was best: 16 GiB map
+// 30.9 ms against 31.2 / 31.8 / 33.2 at 256 / 384 / 512.)
+constexpr uint32_t coop_div(uint32_t segOdds) { return segOdds == 32768u * 8 ? 320u : 768u; }
 
 // Odd primes in (kLargestPatternPrime, x): the cooperative primes of a
-// segment of kCoopDiv * x odd values, counted at compile time to size their
+// segment of coop_div * x odd values, counted at compile time to size their
 // table.
 constexpr uint32_t coop_primes_below(uint32_t x)
 {
@@ -120,7 +126,7 @@ constexpr uint32_t coop_primes_below(uint32_t x)
     return n;
 }
 // table entries for a segment of segOdds odd values, rounded up to 64
-constexpr uint32_t coop_capacity(uint32_t segOdds) { return (coop_primes_below(segOdds / kCoopDiv) + 63) & ~63u; }
+constexpr uint32_t coop_capacity(uint32_t segOdds) { return (coop_primes_below(segOdds / coop_div(segOdds)) + 63) & ~63u; }
 
 struct SieveArgs {
     uint8_t *dst;          // receives map bytes [byteLo, byteHi) at dst[byte - dstBase]
@@ -130,7 +136,7 @@ struct SieveArgs {
     const uint32_t *primes;  // sieving primes > 61, ascending
     const double *invp;      // 1.0 / primes[i]
     uint32_t nprimes;
-    uint32_t ncoop;          // primes[0, ncoop) strike wave-cooperatively (> kCoopDiv strikes per segment)
+    uint32_t ncoop;          // primes[0, ncoop) strike wave-cooperatively (> coop_div strikes per segment)
     uint32_t nwheel;         // primes[0, nwheel) of those skip their multiples of 3 (mod-6 wheel)
     uint32_t nwheel30;       // primes[0, nwheel30) of those also skip their multiples of 5 (mod-30 wheel)
     uint32_t ncarry;         // lane primes carried per thread (<= the kernel's kCarry)
@@ -817,8 +823,9 @@ static int launch(const ffx_sieve_plan *pl, uint8_t *dst, uint64_t dstBase, uint
     a.invp = pl->dInv;
     a.nprimes = pl->nprimes;
     const uint64_t segBytes = pl->segBytes, segOdds = segBytes * 8;
-    // primes with more than kCoopDiv strikes per segment strike wave-cooperatively
-    a.ncoop = (uint32_t)(std::lower_bound(pl->hostPrimes.begin(), pl->hostPrimes.end(), (uint32_t)(segOdds / kCoopDiv)) -
+    // primes with more than coop_div strikes per segment strike wave-cooperatively
+    a.ncoop = (uint32_t)(std::lower_bound(pl->hostPrimes.begin(), pl->hostPrimes.end(),
+                                          (uint32_t)(segOdds / coop_div((uint32_t)segOdds))) -
                          pl->hostPrimes.begin());
     const uint32_t coopCap = segBytes == kSmallMapSegBytes ? coop_capacity(kSmallMapSegBytes * 8) : coop_capacity(kLargeMapSegBytes * 8);
     if (a.ncoop > coopCap) return fail(FFX_EINVAL, "sieve: %u cooperative primes exceed the table", a.ncoop);
