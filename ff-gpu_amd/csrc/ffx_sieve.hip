@@ -10,7 +10,7 @@
 //
 // Kernel design (gfx950, wave64, 160 KiB LDS per CU):
 //  * One workgroup of 1024 threads owns one map segment at a time in LDS
-//    (32 KiB = 2^18 odd values below maps of 2^35 values, 112 KiB from there)
+//    (32 KiB = 2^18 odd values below maps of 2^32 values, 112 KiB from there)
 //    and sieves a contiguous run of segments.
 //  * Bits are kept "struck" = 1 in LDS, in natural order (bit b of word w is
 //    odd index 32w+b); the write-out inverts them and reorders each byte to
@@ -36,9 +36,12 @@
 namespace {
 
 // Defaults measured on MI355X (ff_sieve --autotune and direct timing):
-// maps whose sieving primes stay below 2^17.5 (M < 2^35) are fastest with
+// maps whose sieving primes stay below 2^16 (M < 2^32) are fastest with
 // 32 KiB segments and four 1024-thread workgroups per CU (two resident with
-// the 43 KiB of patterns): 1 GiB map 1.64 ms against 1.70 with 112 KiB.
+// the 43 KiB of patterns): 64 MiB map 0.121 ms against 0.148 with 112 KiB,
+// but 256 MiB 0.404 against 0.348, 1 GiB 1.44 against 1.25 (since the
+// cooperative primes are taken from a counter; before, 1 GiB took 1.64
+// with 32 KiB against 1.70 and the switch was at 2^35 values).
 // Larger maps, where most primes strike a segment only a few times and
 // each segment pays every prime's set-up, are faster with the largest
 // segment the LDS holds beside the patterns, one workgroup resident per
@@ -48,7 +51,7 @@ namespace {
 // CU (one: 16 GiB 28.49 ms, two: 28.30, three: 28.33, four: 28.31).
 constexpr uint32_t kSmallMapSegBytes = 32768, kSmallMapBlocksPerCU = 4;
 constexpr uint32_t kLargeMapSegBytes = 114688, kLargeMapBlocksPerCU = 2;
-constexpr uint64_t kLargeMapFrom = 1ull << 35;
+constexpr uint64_t kLargeMapFrom = 1ull << 32;
 constexpr int kThreads = 1024;
 // Cooperative primes up to segOdds / kWheelDiv strike through a mod-6 wheel
 // (a third fewer LDS atomics; above it a lane has too few strikes per
